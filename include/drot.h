@@ -1,0 +1,173 @@
+/*
+ * drot.h -- C ABI of the MI355X (gfx950) Douglas-Rachford regularized optimal
+ * transport solver, libdrot.so.
+ *
+ * Matrix convention (same as the reference, core/kernel_qr.hpp:3): C and X are
+ * column-major n_rows x n_cols, element (i, j) at [j * n_rows + i], with
+ * n_rows = len(p) and n_cols = len(q).  A row-major PyTorch tensor of shape
+ * [n_cols, n_rows] (= [M, N] in pytorch/src/qr_drot_cuda_kernel.cu:89-93) has
+ * exactly this memory.
+ *
+ * Two families of entry points:
+ *   *_solve_f32    device pointers + caller-owned workspace + stream; what a
+ *                  framework binding calls (replaces the reference's L1 loops
+ *                  quadratic_regularizer_drot / group_lasso_regularizer_drot).
+ *   *_wrapper_f32  host pointers, allocation, copies and timers inside; the
+ *                  reference's core wrappers with the same argument order.
+ * All functions are thread-safe for distinct workspaces.  Calls block until the
+ * solve has finished on the given stream (as the reference's loops do).
+ */
+#ifndef DROT_H_
+#define DROT_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DROT_ABI_VERSION 1
+
+typedef enum drot_status {
+    DROT_OK = 0,
+    DROT_ERR_INVALID_ARGUMENT = 1,   /* bad size, null pointer, bad parameter */
+    DROT_ERR_WORKSPACE_TOO_SMALL = 2,
+    DROT_ERR_UNSUPPORTED = 3,        /* shape outside what the kernels handle */
+    DROT_ERR_HIP = 4,                /* a HIP runtime call failed */
+    DROT_ERR_ALLOC = 5               /* host or device allocation failed */
+} drot_status;
+
+/* QR-DROT parameters.  Mirrors the scalar arguments of
+ * quadratic_regularizer_drot<T> (core/drot_qr.hpp:159-185). */
+typedef struct drot_qr_params {
+    float step_size;  /* s = rho / (n_rows + n_cols)                           */
+    float scale;      /* kappa = 1 + s (n_rows + n_cols) r_weight              */
+    float r_weight;   /* kept for reporting; the iteration uses `scale`       */
+    int max_iters;
+    float eps;        /* stop when the primal residual < eps (even iterations) */
+    int warm_start;   /* 1: closed-form warm start (always used by the PyTorch
+                         binding); 0: X = p q^T (core/drot_qr.hpp:308-320)     */
+} drot_qr_params;
+
+/* GL-DROT parameters.  Mirrors group_lasso_regularizer_drot<T>
+ * (core/drot_glr.hpp:242-273). */
+typedef struct drot_glr_params {
+    float step_size;  /* s = rho / (n_rows + n_cols)                           */
+    float lambda;     /* group soft-threshold (already includes the sqrt factor) */
+    int n_groups;     /* groups are row ranges of each column                  */
+    int max_iters;
+    float eps;        /* stop when the primal residual <= eps (every iteration) */
+    int warm_start;
+} drot_glr_params;
+
+typedef struct drot_result {
+    int n_iter;            /* iteration count in the reference's convention:
+                              QR: k+1 at convergence, max_iters+1 otherwise
+                              (core/drot_qr.hpp:233); GL: executed iterations */
+    int converged;         /* 1 if the stopping test passed                    */
+    int iterations_run;    /* iterations actually executed                     */
+    float objective;       /* <C, X'> of the last iteration that computed it   */
+    double objective_f64;  /* same, fp64 accumulation                          */
+    double residual;       /* sqrt(|rowsum - p|^2 + |colsum - q|^2), fp64, of
+                              the last iteration that ran the stopping test    */
+    float run_ms;          /* wrappers only: solve time (as dur_in_ms)         */
+    float prep_ms;         /* wrappers only: alloc + copies + init             */
+} drot_result;
+
+/* ---------------------------------------------------------------- params */
+
+/* PyTorch binding convention (pytorch/src/qr_drot_cuda_kernel.cu:38-40):
+ * s = rho / (float(n_rows) + float(n_cols)), kappa = 1 + s*r_weight*(n_rows+n_cols),
+ * warm start. */
+drot_status drot_qr_params_from_rho(float rho, float r_weight, int n_rows, int n_cols,
+                                    int max_iters, float eps, drot_qr_params *out);
+
+/* Core wrapper convention (core/drot_qr.hpp:262): s given,
+ * kappa = 1 + s*(n_rows+n_cols)*r_weight. */
+drot_status drot_qr_params_from_step(float step_size, float r_weight, int n_rows,
+                                     int n_cols, int max_iters, float eps,
+                                     int warm_start, drot_qr_params *out);
+
+/* PyTorch binding convention (pytorch/src/glr_drot_cuda_kernel.cu:45-47):
+ * s = rho/(n_rows+n_cols), lambda = r_weight * sqrt(n_rows / n_groups). */
+drot_status drot_glr_params_from_rho(float rho, float r_weight, int n_rows, int n_cols,
+                                     int n_groups, int max_iters, float eps,
+                                     drot_glr_params *out);
+
+/* Core wrapper convention (core/drot_glr.hpp:385):
+ * lambda = r_weight / sqrt(n_rows / n_groups). */
+drot_status drot_glr_params_from_step(float step_size, float r_weight, int n_rows,
+                                      int n_cols, int n_groups, int max_iters,
+                                      float eps, int warm_start, drot_glr_params *out);
+
+/* ---------------------------------------------------------------- device API */
+
+/* Bytes of device workspace a solve of this shape needs (256-byte aligned
+ * start recommended).  Replaces the 11 (QR) / 17 (GL) workspace tensors the
+ * reference binding allocates (pytorch/src/qr_drot_cuda_kernel.cu:107-119,
+ * pytorch/src/glr_drot_cuda_kernel.cu:118-135). */
+drot_status drot_qr_workspace_size(int n_rows, int n_cols, size_t *bytes);
+drot_status drot_glr_workspace_size(int n_rows, int n_cols, int n_groups, size_t *bytes);
+
+/* QR-DROT solve on device memory.  Replaces quadratic_regularizer_drot<float>
+ * (core/drot_qr.hpp:159-234) as called by quadratic_regularizer_drot_torch_float32
+ * (pytorch/src/qr_drot_cuda_kernel.cu:12-78), including its initialisation.
+ * c, p, q, x: device pointers; x receives the plan.  stream: a hipStream_t
+ * (NULL = default stream).  result may be NULL. */
+drot_status drot_qr_solve_f32(const float *c, const float *p, const float *q, int n_rows,
+                              int n_cols, const drot_qr_params *params, float *x,
+                              void *workspace, size_t workspace_bytes, void *stream,
+                              drot_result *result);
+
+/* GL-DROT solve on device memory.  Replaces group_lasso_regularizer_drot<float>
+ * (core/drot_glr.hpp:242-355) as called by group_lasso_regularizer_drot_torch_float32
+ * (pytorch/src/glr_drot_cuda_kernel.cu:13-86). */
+drot_status drot_glr_solve_f32(const float *c, const float *p, const float *q, int n_rows,
+                               int n_cols, const drot_glr_params *params, float *x,
+                               void *workspace, size_t workspace_bytes, void *stream,
+                               drot_result *result);
+
+/* Export the dual state left in `workspace` by the last solve of this shape:
+ * phi1[n_rows], phi2[n_cols] (device pointers, may be NULL), and on the host
+ * alpha (may be NULL).  kind: 0 = QR, 1 = GL.  The reference keeps this state
+ * internal (core/drot_qr.hpp:74-76); the potentials are u = phi1/s, v = phi2/s. */
+drot_status drot_export_duals_f32(int kind, int n_rows, int n_cols, int n_groups,
+                                  const void *workspace, size_t workspace_bytes,
+                                  float *phi1, float *phi2, float *alpha, void *stream);
+
+/* ---------------------------------------------------------------- host API */
+
+/* Host-pointer wrapper with the argument order of
+ * quadratic_regularizer_drot_wrapper<float> (core/drot_qr.hpp:241-257).
+ * step_size is already normalised (rho/(n_rows+n_cols)).  Returns a malloc'd
+ * host copy of X when return_x != 0 (free it with drot_free), else NULL; NULL
+ * also on error (see drot_last_error). */
+float *drot_qr_wrapper_f32(const float *c, const float *p, const float *q, int n_rows,
+                           int n_cols, float step_size, float r_weight, int max_iters,
+                           float eps, float *dur_in_ms, float *prep_dur_in_ms,
+                           int *n_iter, float *objective, int use_warmup_init,
+                           int return_x);
+
+/* Host-pointer wrapper with the argument order of
+ * group_lasso_regularizer_drot_wrapper<float> (core/drot_glr.hpp:362-379);
+ * lambda = r_weight / sqrt(n_rows / n_groups) as in core/drot_glr.hpp:385. */
+float *drot_glr_wrapper_f32(const float *c, const float *p, const float *q, int n_rows,
+                            int n_cols, int n_groups, float step_size, float r_weight,
+                            int max_iters, float eps, float *dur_in_ms,
+                            float *prep_dur_in_ms, int *n_iter, float *objective,
+                            int use_warmup_init, int return_x);
+
+void drot_free(void *ptr);
+
+/* ---------------------------------------------------------------- misc */
+
+const char *drot_status_string(drot_status status);
+/* Message of the last failing call on this thread ("" if none). */
+const char *drot_last_error(void);
+int drot_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DROT_H_ */

@@ -1,0 +1,97 @@
+// Host-visible launch interface of the solver kernels (drot_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "drot_device.h"
+
+namespace drot {
+
+// Arguments of the fused QR pass (one DR iteration over X).
+struct QrPassArgs {
+    float *x;
+    const float *c;
+    const float *phi1;  // explicit phi (SolverState::phi_mode == 1)
+    const float *phi2;
+    const float *ahat;  // phi numerators (phi_mode == 0): phi1 = (ahat + gamma) / n_cols
+    const float *bhat;  //                                 phi2 = (bhat + gamma) / n_rows
+    const SolverState *st;
+    double *prow;       // [n_ct][n_rows] row partial sums
+    double *pcol;       // [n_rt][n_cols] column partial sums
+    double *pobj;       // [n_rt * n_ct] objective partials
+    int n_rows, n_cols;
+    int n_rt, n_ct;     // row tiles, column tiles
+    int nstep;          // column steps per wave; tile columns = nstep * 4 * (64 / lpc)
+    float step, kappa;
+};
+
+// Arguments of the fused GL pass (prox + group soft-threshold + sums).
+struct GlPassArgs {
+    float *x;
+    const float *c;
+    const float *phi1, *phi2, *ahat, *bhat;
+    const SolverState *st;
+    double *prow;       // [n_ct][n_rows]
+    double *pcol;       // [n_cols]
+    double *pobj;       // [n_ct]
+    const int *bounds;  // [n_groups + 1] group row bounds
+    int n_rows, n_cols, n_groups;
+    int n_ct, tc;       // column tiles, columns per tile
+    int lane_per_group; // 1: one lane sums a group serially (small groups)
+    float step, lambda;
+};
+
+// Arguments of the dual / aux update (shared by QR and GL).
+struct FinalizeArgs {
+    const double *prow, *pcol, *pobj;
+    const float *p, *q;
+    float *a, *b, *ahat, *bhat;
+    double *pfin;       // [rb + cb][2]
+    SolverState *st;
+    int n_rows, n_cols, n_rt, n_ct;
+    int n_obj;          // number of objective partials
+    int rb, cb;         // row blocks, column blocks
+    int max_iters;
+    float eps;
+    int stop_rule;      // StopRule
+    int obj_even_only;  // 1 for QR (objective on even iterations only)
+};
+
+struct InitArgs {
+    float *x;
+    const float *p, *q;
+    float *phi1, *phi2, *a, *b, *ahat, *bhat;
+    SolverState *st;
+    int n_rows, n_cols;
+    int warm;
+    float v_phi1, v_phi2, v_a, v_b, v_alpha;
+};
+
+struct QrTile {
+    int vec;   // 1 or 4 floats per lane access
+    int lpc;   // lanes per column: 16, 32 or 64
+};
+
+struct GlTile {
+    int vec;   // 1 or 4
+    int rch;   // row chunks per thread
+};
+
+hipError_t launch_init(const InitArgs &a, hipStream_t s);
+hipError_t launch_qr_pass(const QrTile &t, bool even, int grid, const QrPassArgs &a,
+                          hipStream_t s);
+hipError_t launch_gl_pass(const GlTile &t, int grid, size_t lds_bytes,
+                          const GlPassArgs &a, hipStream_t s);
+hipError_t launch_finalize(int grid, const FinalizeArgs &a, hipStream_t s);
+hipError_t launch_fix_x(float *x, const float *c, float step, size_t total,
+                        const SolverState *st, hipStream_t s);
+hipError_t launch_export_phi(const float *ahat, const float *bhat, const float *phi1,
+                             const float *phi2, const SolverState *st, int n_rows,
+                             int n_cols, float *out_phi1, float *out_phi2,
+                             hipStream_t s);
+
+bool qr_tile_supported(const QrTile &t);
+bool gl_tile_supported(const GlTile &t);
+
+}  // namespace drot

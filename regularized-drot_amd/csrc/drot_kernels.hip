@@ -1,0 +1,597 @@
+// gfx950 kernels of the DR regularized-OT solver.
+//
+// One DR iteration = one fused pass over X (and C) + one small finalize launch:
+//   qr_pass / gl_pass : X-update (prox), and in the same pass the contiguous-axis
+//                       sums (wave butterfly over the lanes of one column), the
+//                       strided-axis sums (per-lane fp64 accumulators combined
+//                       through LDS into one partial per tile) and the objective.
+//   finalize          : fixed-order reduction of the tile partials, the dual /
+//                       aux update of core/kernel_qr.hpp:405-467, the residual
+//                       and the stopping test -- all on device, no host round trip.
+// The pass is HBM-bound elementwise work (about 1 flop per byte), so it uses
+// plain VALU, 128-bit coalesced accesses and no matrix cores.
+#include "drot_kernels.h"
+
+#include <math.h>
+
+namespace drot {
+
+// ----------------------------------------------------------------- helpers
+
+template <int VEC>
+struct VecIO;
+
+template <>
+struct VecIO<4> {
+    static __device__ __forceinline__ void load(const float *p, float (&v)[4]) {
+        const float4 t = *reinterpret_cast<const float4 *>(p);
+        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    }
+    static __device__ __forceinline__ void store(float *p, const float (&v)[4]) {
+        *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    }
+};
+
+template <>
+struct VecIO<1> {
+    static __device__ __forceinline__ void load(const float *p, float (&v)[1]) { v[0] = *p; }
+    static __device__ __forceinline__ void store(float *p, const float (&v)[1]) { *p = v[0]; }
+};
+
+// phi of one row / column: explicit array on the first iteration after an
+// initialisation, (hat + gamma) / n afterwards.  The two fp32 roundings are
+// those of `(ab_val - rs_val + gamma) / T(n)` in core/kernel_qr.hpp:431,448
+// (the `ab_val - rs_val` part is stored in hat by the finalize kernel).
+__device__ __forceinline__ float phi_of(int mode, const float *phi, const float *hat,
+                                        int idx, float gamma, float n) {
+    return mode ? phi[idx] : (hat[idx] + gamma) / n;
+}
+
+// ----------------------------------------------------------------- init
+
+// Warm start (pytorch/src/qr_drot_cuda_kernel.cu:45-71: X = 0 and constant
+// phi/a/b/alpha computed on the host) or cold start (core/drot_qr.hpp:308-320:
+// X = p q^T, phi = a = b = 0).  Fill kernels only, no memset nodes.
+__global__ __launch_bounds__(kBlock) void init_kernel(InitArgs A) {
+    const size_t total = (size_t)A.n_rows * (size_t)A.n_cols;
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nthr = (size_t)gridDim.x * blockDim.x;
+    if (A.warm) {
+        for (size_t e = tid; e < total; e += nthr) A.x[e] = 0.0f;
+    } else {
+        for (size_t e = tid; e < total; e += nthr) {
+            const int j = (int)(e / (size_t)A.n_rows);
+            const int i = (int)(e - (size_t)j * (size_t)A.n_rows);
+            A.x[e] = A.p[i] * A.q[j];
+        }
+    }
+    for (size_t i = tid; i < (size_t)A.n_rows; i += nthr) {
+        A.phi1[i] = A.warm ? A.v_phi1 : 0.0f;
+        A.a[i] = A.warm ? A.v_a : 0.0f;
+        A.ahat[i] = 0.0f;
+    }
+    for (size_t j = tid; j < (size_t)A.n_cols; j += nthr) {
+        A.phi2[j] = A.warm ? A.v_phi2 : 0.0f;
+        A.b[j] = A.warm ? A.v_b : 0.0f;
+        A.bhat[j] = 0.0f;
+    }
+    if (tid == 0) {
+        SolverState s;
+        s.k = 0;
+        s.done = 0;
+        s.converged = 0;
+        s.phi_mode = 1;
+        s.gamma = 0.0f;
+        s.alpha = A.warm ? A.v_alpha : 0.0f;
+        s.n_iter = 0;
+        s.counter = 0u;
+        s.obj = 0.0;
+        s.res = 1e9;
+        s.xsum = 0.0;
+        s.res_last = 1e9;
+        s.obj_last = 0.0;
+        s.last_k = -1;
+        s.pad_[0] = s.pad_[1] = s.pad_[2] = 0;
+        *A.st = s;
+    }
+}
+
+// ----------------------------------------------------------------- QR pass
+
+// Fused QR-DROT pass, restating quadratic_regularizer_update_x_{even,odd}
+// (core/kernel_qr.hpp:171-376) for one tile of TR = LPC*VEC rows by
+// nstep*4*(64/LPC) columns:
+//   X' = max(X + phi1 + phi2 - s C, 0) / kappa      (EVEN; ODD drops the -sC,
+//                                                    which is already in X)
+//   stored X = X' - s C (EVEN) | X' (ODD)
+// Lane layout: LPC lanes cover VEC*LPC contiguous rows of one column, so a wave
+// handles 64/LPC columns at once and every access is a coalesced 16-byte load.
+// The column (contiguous-axis) partial is a butterfly over those LPC lanes; the
+// row (strided-axis) partials live in per-lane fp64 registers across the tile's
+// columns and are combined through LDS in a fixed order at the end.
+template <int VEC, int LPC, bool EVEN>
+__global__ __launch_bounds__(kBlock) void qr_pass_kernel(QrPassArgs A) {
+    constexpr int CPW = kWave / LPC;  // columns per wave step
+    constexpr int TR = LPC * VEC;     // tile rows
+    constexpr int U = 4;              // column steps in flight per lane
+    __shared__ double s_row[kWavesPerBlock * CPW][TR];
+    __shared__ double s_red[kWavesPerBlock];
+
+    const SolverState *st = A.st;
+    if (st->done) return;
+    const int phi_mode = st->phi_mode;
+    const float gamma = st->gamma;
+
+    const int N = A.n_rows, M = A.n_cols;
+    const int rt = blockIdx.x % A.n_rt;
+    const int ct = blockIdx.x / A.n_rt;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = wave_uniform(threadIdx.x / kWave);
+    const int cs = lane / LPC;
+    const int rl = lane % LPC;
+    const int row = rt * TR + rl * VEC;
+    const bool row_ok = row < N;  // VEC == 4 only when N % 4 == 0
+    const int tc = A.nstep * kWavesPerBlock * CPW;
+    const int col0 = ct * tc;
+    const float fM = (float)M, fN = (float)N;
+    const float s = A.step, kappa = A.kappa;
+
+    float f1[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; e++)
+        f1[e] = row_ok ? phi_of(phi_mode, A.phi1, A.ahat, row + e, gamma, fM) : 0.0f;
+
+    double racc[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; e++) racc[e] = 0.0;
+    double oacc = 0.0;
+
+    for (int t0 = 0; t0 < A.nstep; t0 += U) {
+        float xv[U][VEC], cv[U][VEC];
+        int jj[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u;
+            jj[u] = col0 + (t * kWavesPerBlock + w) * CPW + cs;
+            ok[u] = (t < A.nstep) && (jj[u] < M) && row_ok;
+            if (ok[u]) {
+                const size_t off = (size_t)jj[u] * (size_t)N + (size_t)row;
+                VecIO<VEC>::load(A.x + off, xv[u]);
+                if (EVEN) VecIO<VEC>::load(A.c + off, cv[u]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < VEC; e++) { xv[u][e] = 0.0f; cv[u][e] = 0.0f; }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u;
+            const int j = jj[u];
+            const bool col_ok = (t < A.nstep) && (j < M);
+            const float f2 = col_ok ? phi_of(phi_mode, A.phi2, A.bhat, j, gamma, fN) : 0.0f;
+            float xo[VEC];
+            double csum = 0.0;
+#pragma unroll
+            for (int e = 0; e < VEC; e++) {
+                const float tt = (xv[u][e] + f1[e]) + f2;
+                float y = EVEN ? __builtin_fmaf(-s, cv[u][e], tt) : tt;
+                y = fmaxf(y, 0.0f);
+                const float xp = y / kappa;
+                xo[e] = EVEN ? __builtin_fmaf(-s, cv[u][e], xp) : xp;
+                if (ok[u]) {
+                    racc[e] += (double)xp;
+                    csum += (double)xp;
+                    if (EVEN) oacc += (double)xp * (double)cv[u][e];
+                }
+            }
+            if (ok[u]) {
+                const size_t off = (size_t)j * (size_t)N + (size_t)row;
+                VecIO<VEC>::store(A.x + off, xo);
+            }
+            if (t0 + u < A.nstep) {  // wave-uniform
+                csum = seg_sum<LPC>(csum);
+                if (rl == 0 && j < M) A.pcol[(size_t)rt * M + j] = csum;
+            }
+        }
+    }
+
+    // strided-axis partials: fixed-order combine of the 4*CPW column slots
+    const int slot = w * CPW + cs;
+#pragma unroll
+    for (int e = 0; e < VEC; e++) s_row[slot][rl * VEC + e] = racc[e];
+    __syncthreads();
+    for (int r = threadIdx.x; r < TR; r += kBlock) {
+        double sum = 0.0;
+#pragma unroll
+        for (int k = 0; k < kWavesPerBlock * CPW; k++) sum += s_row[k][r];
+        const int i = rt * TR + r;
+        if (i < N) A.prow[(size_t)ct * N + i] = sum;
+    }
+    if (EVEN) {
+        const double o = block_sum(oacc, s_red);
+        if (threadIdx.x == 0) A.pobj[blockIdx.x] = o;
+    }
+}
+
+// ----------------------------------------------------------------- GL pass
+
+// Fused GL-DROT pass for one tile of `tc` whole columns.  Restates
+// group_lasso_regularizer_update_x_for_small (core/kernel_glr.hpp:59-161) and
+// the large-group chain update_x / update_norm / apply_norm
+// (core/kernel_glr.hpp:163-399) in ONE pass: the block holds the whole column
+// in registers, so the group norms are formed before X is written and the
+// reference's second apply_norm pass over X disappears.
+//   X' = max(X + phi1 + phi2 - s C, 0)
+//   scale_{g,j} = max(1 - lambda / (||X'_{g,j}||_2 + 1e-9), 0)
+//   X_{g,j} := X'_{g,j} * scale_{g,j}
+// Thread layout: row chunk r of thread t covers rows (r*kBlock + t)*VEC ...+VEC-1.
+template <int VEC, int RCH>
+__global__ __launch_bounds__(kBlock) void gl_pass_kernel(GlPassArgs A) {
+    extern __shared__ double s_dyn[];
+    __shared__ double s_red[kWavesPerBlock];
+    const SolverState *st = A.st;
+    if (st->done) return;
+    const int phi_mode = st->phi_mode;
+    const float gamma = st->gamma;
+    const int N = A.n_rows, M = A.n_cols, G = A.n_groups;
+    double *s_sq = s_dyn;                                   // [N]
+    float *s_scale = reinterpret_cast<float *>(s_dyn + N);  // [G]
+    const float fM = (float)M, fN = (float)N;
+    const float s = A.step;
+    const double lam = (double)A.lambda;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = wave_uniform(threadIdx.x / kWave);
+
+    float f1[RCH][VEC];
+    int gid[RCH][VEC];
+    double racc[RCH][VEC];
+#pragma unroll
+    for (int r = 0; r < RCH; r++) {
+#pragma unroll
+        for (int e = 0; e < VEC; e++) {
+            const int i = (r * kBlock + (int)threadIdx.x) * VEC + e;
+            racc[r][e] = 0.0;
+            f1[r][e] = 0.0f;
+            gid[r][e] = 0;
+            if (i < N) {
+                f1[r][e] = phi_of(phi_mode, A.phi1, A.ahat, i, gamma, fM);
+                // group of row i: largest g with bounds[g] <= i
+                int lo = 0, hi = G - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (A.bounds[mid] <= i) lo = mid; else hi = mid - 1;
+                }
+                gid[r][e] = lo;
+            }
+        }
+    }
+    double oacc = 0.0;
+
+    const int j0 = blockIdx.x * A.tc;
+    const int j1 = min(j0 + A.tc, M);
+    for (int j = j0; j < j1; j++) {
+        const float f2 = phi_of(phi_mode, A.phi2, A.bhat, j, gamma, fN);
+        const float *xc = A.x + (size_t)j * N;
+        const float *cc = A.c + (size_t)j * N;
+        float xv[RCH][VEC], cv[RCH][VEC];
+#pragma unroll
+        for (int r = 0; r < RCH; r++) {
+            const int i0 = (r * kBlock + (int)threadIdx.x) * VEC;
+            if (i0 < N) {
+                VecIO<VEC>::load(xc + i0, xv[r]);
+                VecIO<VEC>::load(cc + i0, cv[r]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < VEC; e++) { xv[r][e] = 0.0f; cv[r][e] = 0.0f; }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RCH; r++) {
+            const int i0 = (r * kBlock + (int)threadIdx.x) * VEC;
+#pragma unroll
+            for (int e = 0; e < VEC; e++) {
+                const float tt = (xv[r][e] + f1[r][e]) + f2;
+                const float y = fmaxf(__builtin_fmaf(-s, cv[r][e], tt), 0.0f);
+                xv[r][e] = y;
+                if (i0 + e < N) s_sq[i0 + e] = (double)y * (double)y;
+            }
+        }
+        __syncthreads();
+        // group norms -> scales, fixed order
+        if (A.lane_per_group) {
+            for (int g = threadIdx.x; g < G; g += kBlock) {
+                double n2 = 0.0;
+                for (int i = A.bounds[g]; i < A.bounds[g + 1]; i++) n2 += s_sq[i];
+                const double sc = 1.0 - lam / (sqrt(n2) + 1e-9);
+                s_scale[g] = (float)(sc > 0.0 ? sc : 0.0);
+            }
+        } else {
+            for (int g = w; g < G; g += kWavesPerBlock) {
+                double n2 = 0.0;
+                for (int i = A.bounds[g] + lane; i < A.bounds[g + 1]; i += kWave) n2 += s_sq[i];
+                n2 = wave_sum(n2);
+                if (lane == 0) {
+                    const double sc = 1.0 - lam / (sqrt(n2) + 1e-9);
+                    s_scale[g] = (float)(sc > 0.0 ? sc : 0.0);
+                }
+            }
+        }
+        __syncthreads();
+        double csum = 0.0;
+#pragma unroll
+        for (int r = 0; r < RCH; r++) {
+            const int i0 = (r * kBlock + (int)threadIdx.x) * VEC;
+            if (i0 < N) {
+                float xo[VEC];
+#pragma unroll
+                for (int e = 0; e < VEC; e++) {
+                    const float v = xv[r][e] * s_scale[gid[r][e]];
+                    xo[e] = v;
+                    racc[r][e] += (double)v;
+                    csum += (double)v;
+                    oacc += (double)v * (double)cv[r][e];
+                }
+                VecIO<VEC>::store(A.x + (size_t)j * N + i0, xo);
+            }
+        }
+        csum = block_sum(csum, s_red);  // also fences s_sq / s_scale reuse
+        if (threadIdx.x == 0) A.pcol[j] = csum;
+    }
+#pragma unroll
+    for (int r = 0; r < RCH; r++) {
+#pragma unroll
+        for (int e = 0; e < VEC; e++) {
+            const int i = (r * kBlock + (int)threadIdx.x) * VEC + e;
+            if (i < N) A.prow[(size_t)blockIdx.x * N + i] = racc[r][e];
+        }
+    }
+    const double o = block_sum(oacc, s_red);
+    if (threadIdx.x == 0) A.pobj[blockIdx.x] = o;
+}
+
+// ----------------------------------------------------------------- finalize
+
+// Dual / aux update shared by QR and GL: quadratic_regularizer_update_aux +
+// update_alpha + the residual reductions (core/kernel_qr.hpp:405-467,
+// core/drot_qr.hpp:145-156; identical in core/kernel_glr.hpp:461-526):
+//   rowsum = sum_ct prow ; rs = rowsum - p ; a <- a - rs ; ahat = a - rs
+//   colsum = sum_rt pcol ; cs = colsum - q ; b <- b - cs ; bhat = b - cs
+// Blocks [0, rb) own rows, [rb, rb+cb) own columns.  The last block to arrive
+// forms S = sum X', gamma = (2(S-1) - alpha)/(n_rows+n_cols) (old alpha),
+// alpha <- (alpha+1) - S, the residual and the stopping test.  phi itself is
+// formed by the next pass as (hat + gamma)/n, so no second grid-wide step is
+// needed here.
+__global__ __launch_bounds__(kBlock) void finalize_kernel(FinalizeArgs A) {
+    __shared__ double s_red[kWavesPerBlock];
+    __shared__ int s_last;
+    SolverState *st = A.st;
+    if (st->done) return;
+    const int N = A.n_rows, M = A.n_cols;
+    const int b = blockIdx.x;
+    double res = 0.0, S = 0.0;
+    if (b < A.rb) {
+        const int i = b * kBlock + threadIdx.x;
+        if (i < N) {
+            const double *src = A.prow + i;
+            double sum = 0.0;
+            int ct = 0;
+            for (; ct + 8 <= A.n_ct; ct += 8) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) v[u] = src[(size_t)(ct + u) * N];
+#pragma unroll
+                for (int u = 0; u < 8; u++) sum += v[u];
+            }
+            for (; ct < A.n_ct; ct++) sum += src[(size_t)ct * N];
+            const double d = sum - (double)A.p[i];
+            const float rs = (float)d;
+            const float an = A.a[i] - rs;
+            A.a[i] = an;
+            A.ahat[i] = an - rs;
+            res = d * d;
+        }
+    } else {
+        const int j = (b - A.rb) * kBlock + threadIdx.x;
+        if (j < M) {
+            const double *src = A.pcol + j;
+            double sum = 0.0;
+            int rt = 0;
+            for (; rt + 8 <= A.n_rt; rt += 8) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) v[u] = src[(size_t)(rt + u) * M];
+#pragma unroll
+                for (int u = 0; u < 8; u++) sum += v[u];
+            }
+            for (; rt < A.n_rt; rt++) sum += src[(size_t)rt * M];
+            const double d = sum - (double)A.q[j];
+            const float cs = (float)d;
+            const float bn = A.b[j] - cs;
+            A.b[j] = bn;
+            A.bhat[j] = bn - cs;
+            res = d * d;
+            S = sum;
+        }
+    }
+    res = block_sum(res, s_red);
+    S = block_sum(S, s_red);
+    if (threadIdx.x == 0) {
+        A.pfin[2 * b] = res;
+        A.pfin[2 * b + 1] = S;
+        __threadfence();
+        const unsigned int prev = atomicAdd(&st->counter, 1u);
+        s_last = (prev == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+
+    const int k = st->k;
+    const bool even = (k % 2) == 0;
+    const bool has_obj = !A.obj_even_only || even;
+    double rr = 0.0, rc = 0.0, SS = 0.0, obj = 0.0;
+    for (int bb = threadIdx.x; bb < A.rb; bb += kBlock) rr += A.pfin[2 * bb];
+    for (int bb = A.rb + threadIdx.x; bb < A.rb + A.cb; bb += kBlock) {
+        rc += A.pfin[2 * bb];
+        SS += A.pfin[2 * bb + 1];
+    }
+    if (has_obj)
+        for (int bb = threadIdx.x; bb < A.n_obj; bb += kBlock) obj += A.pobj[bb];
+    rr = block_sum(rr, s_red);
+    rc = block_sum(rc, s_red);
+    SS = block_sum(SS, s_red);
+    obj = block_sum(obj, s_red);
+    if (threadIdx.x == 0) {
+        const float alpha = st->alpha;
+        st->gamma = (float)((2.0 * (SS - 1.0) - (double)alpha) / (double)(N + M));
+        st->alpha = (float)(((double)alpha + 1.0) - SS);
+        st->phi_mode = 0;
+        st->xsum = SS;
+        const double resid = sqrt(rr + rc);
+        st->res_last = resid;
+        st->obj_last = has_obj ? obj : 0.0;
+        st->last_k = k;
+        const bool check = (A.stop_rule == kStopEveryLeq) || even;
+        if (check) st->res = resid;
+        if (has_obj) st->obj = obj;
+        const bool conv = check && ((A.stop_rule == kStopEveryLeq)
+                                        ? (resid <= (double)A.eps)
+                                        : (resid < (double)A.eps));
+        if (conv) {
+            st->converged = 1;
+            st->done = 1;
+            st->n_iter = k + 1;
+        } else if (k + 1 >= A.max_iters) {
+            st->done = 1;
+            st->n_iter = (A.stop_rule == kStopEveryLeq) ? A.max_iters : A.max_iters + 1;
+        }
+        st->k = k + 1;
+        st->counter = 0u;
+    }
+}
+
+// ----------------------------------------------------------------- fix_x
+
+// core/kernel_qr.hpp:469-482: undo the -sC shift once a QR solve has converged
+// (convergence is only tested on even iterations, whose stored X is X' - sC).
+__global__ __launch_bounds__(kBlock) void fix_x_kernel(float *x, const float *c, float step,
+                                                      size_t total, const SolverState *st) {
+    if (!st->converged) return;
+    const size_t nthr = (size_t)gridDim.x * blockDim.x;
+    const size_t n4 = total / 4;
+    const bool aligned = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(c)) & 15u) == 0;
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (aligned) {
+        for (; e < n4; e += nthr) {
+            float4 xv = reinterpret_cast<float4 *>(x)[e];
+            const float4 cv = reinterpret_cast<const float4 *>(c)[e];
+            xv.x = fmaxf(__builtin_fmaf(step, cv.x, xv.x), 0.0f);
+            xv.y = fmaxf(__builtin_fmaf(step, cv.y, xv.y), 0.0f);
+            xv.z = fmaxf(__builtin_fmaf(step, cv.z, xv.z), 0.0f);
+            xv.w = fmaxf(__builtin_fmaf(step, cv.w, xv.w), 0.0f);
+            reinterpret_cast<float4 *>(x)[e] = xv;
+        }
+        e = n4 * 4 + ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
+    }
+    for (; e < total; e += nthr) x[e] = fmaxf(__builtin_fmaf(step, c[e], x[e]), 0.0f);
+}
+
+// Materialise phi1/phi2 of the current state (for state export / resume).
+__global__ __launch_bounds__(kBlock) void export_phi_kernel(const float *ahat, const float *bhat,
+                                                           const float *phi1, const float *phi2,
+                                                           const SolverState *st, int n_rows,
+                                                           int n_cols, float *o1, float *o2) {
+    const int mode = st->phi_mode;
+    const float gamma = st->gamma;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nthr = gridDim.x * blockDim.x;
+    for (int i = tid; i < n_rows; i += nthr) o1[i] = phi_of(mode, phi1, ahat, i, gamma, (float)n_cols);
+    for (int j = tid; j < n_cols; j += nthr) o2[j] = phi_of(mode, phi2, bhat, j, gamma, (float)n_rows);
+}
+
+// ----------------------------------------------------------------- launchers
+
+static int grid_for(size_t work, int per_block_items, int cap) {
+    size_t g = (work + (size_t)per_block_items - 1) / (size_t)per_block_items;
+    if (g < 1) g = 1;
+    if (g > (size_t)cap) g = (size_t)cap;
+    return (int)g;
+}
+
+hipError_t launch_init(const InitArgs &a, hipStream_t s) {
+    const size_t total = (size_t)a.n_rows * (size_t)a.n_cols;
+    const int grid = grid_for(total, kBlock * 8, 8192);
+    hipLaunchKernelGGL(init_kernel, dim3(grid), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+bool qr_tile_supported(const QrTile &t) {
+    return (t.vec == 1 || t.vec == 4) && (t.lpc == 16 || t.lpc == 32 || t.lpc == 64);
+}
+
+template <int VEC, int LPC>
+static void launch_qr_t(bool even, int grid, const QrPassArgs &a, hipStream_t s) {
+    if (even)
+        hipLaunchKernelGGL((qr_pass_kernel<VEC, LPC, true>), dim3(grid), dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((qr_pass_kernel<VEC, LPC, false>), dim3(grid), dim3(kBlock), 0, s, a);
+}
+
+hipError_t launch_qr_pass(const QrTile &t, bool even, int grid, const QrPassArgs &a,
+                          hipStream_t s) {
+    if (t.vec == 4) {
+        if (t.lpc == 16) launch_qr_t<4, 16>(even, grid, a, s);
+        else if (t.lpc == 32) launch_qr_t<4, 32>(even, grid, a, s);
+        else launch_qr_t<4, 64>(even, grid, a, s);
+    } else {
+        if (t.lpc == 16) launch_qr_t<1, 16>(even, grid, a, s);
+        else if (t.lpc == 32) launch_qr_t<1, 32>(even, grid, a, s);
+        else launch_qr_t<1, 64>(even, grid, a, s);
+    }
+    return hipGetLastError();
+}
+
+bool gl_tile_supported(const GlTile &t) {
+    if (t.vec == 4) return t.rch == 1 || t.rch == 2 || t.rch == 4 || t.rch == 8;
+    if (t.vec == 1) return t.rch == 1 || t.rch == 2 || t.rch == 4 || t.rch == 8 ||
+                           t.rch == 16 || t.rch == 32;
+    return false;
+}
+
+hipError_t launch_gl_pass(const GlTile &t, int grid, size_t lds, const GlPassArgs &a,
+                          hipStream_t s) {
+#define DROT_GL_CASE(V, R)                                                              \
+    if (t.vec == V && t.rch == R) {                                                     \
+        hipLaunchKernelGGL((gl_pass_kernel<V, R>), dim3(grid), dim3(kBlock), lds, s, a); \
+        return hipGetLastError();                                                       \
+    }
+    DROT_GL_CASE(4, 1) DROT_GL_CASE(4, 2) DROT_GL_CASE(4, 4) DROT_GL_CASE(4, 8)
+    DROT_GL_CASE(1, 1) DROT_GL_CASE(1, 2) DROT_GL_CASE(1, 4) DROT_GL_CASE(1, 8)
+    DROT_GL_CASE(1, 16) DROT_GL_CASE(1, 32)
+#undef DROT_GL_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_finalize(int grid, const FinalizeArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(finalize_kernel, dim3(grid), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fix_x(float *x, const float *c, float step, size_t total,
+                        const SolverState *st, hipStream_t s) {
+    const int grid = grid_for(total / 4 + 1, kBlock * 4, 8192);
+    hipLaunchKernelGGL(fix_x_kernel, dim3(grid), dim3(kBlock), 0, s, x, c, step, total, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_export_phi(const float *ahat, const float *bhat, const float *phi1,
+                             const float *phi2, const SolverState *st, int n_rows,
+                             int n_cols, float *o1, float *o2, hipStream_t s) {
+    const int grid = grid_for((size_t)(n_rows > n_cols ? n_rows : n_cols), kBlock, 1024);
+    hipLaunchKernelGGL(export_phi_kernel, dim3(grid), dim3(kBlock), 0, s, ahat, bhat, phi1,
+                       phi2, st, n_rows, n_cols, o1, o2);
+    return hipGetLastError();
+}
+
+}  // namespace drot

@@ -1,0 +1,740 @@
+// Host side of the solver: tile planning, workspace layout, the iteration
+// driver and the C ABI declared in include/drot.h.
+//
+// The iteration driver replaces the reference's host loops
+// (core/drot_qr.hpp:190-232, core/drot_glr.hpp:278-353).  Those issue 9-16
+// kernels and 3-5 memsets per iteration and block on a D2H copy of aux every
+// one or two iterations.  Here one iteration is two launches (fused pass +
+// finalize) and the stopping test runs on device: the host enqueues iterations
+// in chunks, and every finished chunk is answered by an asynchronous 4-byte
+// snapshot of the device `done` flag that the host reads one chunk later, so
+// the GPU queue never drains while the host polls.  Launches enqueued after
+// convergence return at their first instruction.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/drot.h"
+#include "drot_kernels.h"
+
+namespace drot {
+namespace {
+
+thread_local std::string g_last_error;
+
+drot_status fail(drot_status st, const std::string &msg) {
+    g_last_error = msg;
+    return st;
+}
+
+#define DROT_HIP_CHECK(expr)                                                          \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(DROT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+enum Kind { kQR = 0, kGL = 1 };
+
+// Everything a solve of one shape needs to know about tiles and workspace.
+struct Plan {
+    int kind = kQR;
+    int n_rows = 0, n_cols = 0, n_groups = 1;
+    QrTile qt{4, 32};
+    GlTile gt{4, 1};
+    int n_rt = 1, n_ct = 1, nstep = 1, tc = 1;
+    int grid_pass = 1, n_obj = 1;
+    int rb = 1, cb = 1;
+    int lane_per_group = 0;
+    size_t lds = 0;
+    size_t off_state = 0, off_phi1 = 0, off_phi2 = 0, off_a = 0, off_b = 0;
+    size_t off_ahat = 0, off_bhat = 0, off_prow = 0, off_pcol = 0, off_pobj = 0;
+    size_t off_pfin = 0, off_bounds = 0, total = 0;
+};
+
+// Optional override of the QR tile, "vec,lpc,nstep" (tuning sweeps only).
+bool qr_tile_from_env(int vec_allowed, QrTile *t, int *nstep) {
+    const char *s = getenv("DROT_QR_TILE");
+    if (!s || !*s) return false;
+    int v = 0, l = 0, n = 0;
+    if (sscanf(s, "%d,%d,%d", &v, &l, &n) != 3) return false;
+    QrTile cand{v, l};
+    if (!qr_tile_supported(cand) || n < 1) return false;
+    if (v == 4 && !vec_allowed) return false;
+    *t = cand;
+    *nstep = n;
+    return true;
+}
+
+void layout(Plan &P) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = align_up(off + bytes, 256);
+        return o;
+    };
+    const size_t N = (size_t)P.n_rows, M = (size_t)P.n_cols;
+    P.off_state = take(sizeof(SolverState));
+    P.off_phi1 = take(N * 4);
+    P.off_phi2 = take(M * 4);
+    P.off_a = take(N * 4);
+    P.off_b = take(M * 4);
+    P.off_ahat = take(N * 4);
+    P.off_bhat = take(M * 4);
+    P.off_prow = take((size_t)P.n_ct * N * 8);
+    P.off_pcol = take((size_t)P.n_rt * M * 8);
+    P.off_pobj = take((size_t)P.n_obj * 8);
+    P.off_pfin = take((size_t)(P.rb + P.cb) * 16);
+    P.off_bounds = take((size_t)(P.n_groups + 1) * 4);
+    P.total = off;
+}
+
+// QR tile choice.  The pass wants ~1000 resident blocks (256 CUs, 4 waves per
+// block) and few strided-axis partials (one fp64 per row per column tile).
+Plan plan_qr(int n_rows, int n_cols, bool vec4_ok) {
+    Plan P;
+    P.kind = kQR;
+    P.n_rows = n_rows;
+    P.n_cols = n_cols;
+    const bool v4 = vec4_ok && (n_rows % 4 == 0);
+    int nstep = 0;
+    if (!qr_tile_from_env(v4, &P.qt, &nstep)) {
+        if (v4) P.qt = QrTile{4, n_rows >= 1024 ? 32 : 16};
+        else P.qt = QrTile{1, n_rows >= 256 ? 64 : (n_rows >= 32 ? 32 : 16)};
+    }
+    const int tr = P.qt.lpc * P.qt.vec;
+    const int cpw = kWave / P.qt.lpc;
+    const int tc_min = kWavesPerBlock * cpw;
+    P.n_rt = ceil_div(n_rows, tr);
+    if (nstep <= 0) {
+        const int target_blocks = 1024;
+        const int n_ct_target = std::max(1, target_blocks / P.n_rt);
+        const int cols_per = ceil_div(n_cols, n_ct_target);
+        nstep = std::max(1, ceil_div(cols_per, tc_min));
+    }
+    P.nstep = nstep;
+    P.tc = nstep * tc_min;
+    P.n_ct = ceil_div(n_cols, P.tc);
+    P.grid_pass = P.n_rt * P.n_ct;
+    P.n_obj = P.grid_pass;
+    P.rb = ceil_div(n_rows, kBlock);
+    P.cb = ceil_div(n_cols, kBlock);
+    layout(P);
+    return P;
+}
+
+constexpr size_t kGlLdsLimit = 64 * 1024 - 256;  // leaves room for static LDS
+
+// GL tile choice: a block owns whole columns (all row groups of a column) so
+// the group norms are complete before X is written.
+bool plan_gl(int n_rows, int n_cols, int n_groups, bool vec4_ok, Plan *out,
+             std::string *why) {
+    Plan P;
+    P.kind = kGL;
+    P.n_rows = n_rows;
+    P.n_cols = n_cols;
+    P.n_groups = n_groups;
+    const bool v4 = vec4_ok && (n_rows % 4 == 0);
+    P.gt.vec = v4 ? 4 : 1;
+    const int maxr = v4 ? 8 : 32;
+    int rch = 1;
+    while (rch < maxr && kBlock * P.gt.vec * rch < n_rows) rch *= 2;
+    if (kBlock * P.gt.vec * rch < n_rows) {
+        *why = "group-lasso solver supports n_rows <= 8192 (got " + std::to_string(n_rows) + ")";
+        return false;
+    }
+    P.gt.rch = rch;
+    P.lds = (size_t)n_rows * 8 + (size_t)n_groups * 4 + 16;
+    if (P.lds > kGlLdsLimit) {
+        *why = "group-lasso LDS tile exceeds 64 KiB for n_rows=" + std::to_string(n_rows) +
+               ", n_groups=" + std::to_string(n_groups);
+        return false;
+    }
+    P.tc = std::max(1, ceil_div(n_cols, 256));
+    P.n_ct = ceil_div(n_cols, P.tc);
+    P.n_rt = 1;
+    P.grid_pass = P.n_ct;
+    P.n_obj = P.n_ct;
+    P.lane_per_group = (n_rows / std::max(1, n_groups)) < 64 ? 1 : 0;
+    P.rb = ceil_div(n_rows, kBlock);
+    P.cb = ceil_div(n_cols, kBlock);
+    layout(P);
+    *out = P;
+    return true;
+}
+
+bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+// Per-thread pinned snapshot slots and events for the asynchronous done-flag poll.
+struct PollCtx {
+    int *pinned = nullptr;   // 2 done flags + a full SolverState copy
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int device = -1;
+};
+thread_local PollCtx g_poll;
+
+drot_status poll_ctx(PollCtx **out) {
+    int dev = 0;
+    DROT_HIP_CHECK(hipGetDevice(&dev));
+    PollCtx &c = g_poll;
+    if (c.pinned == nullptr || c.device != dev) {
+        // Contexts of a previous device are kept alive (process lifetime).
+        c = PollCtx();
+        DROT_HIP_CHECK(hipHostMalloc((void **)&c.pinned, 4096, hipHostMallocDefault));
+        DROT_HIP_CHECK(hipEventCreateWithFlags(&c.ev[0], hipEventDisableTiming));
+        DROT_HIP_CHECK(hipEventCreateWithFlags(&c.ev[1], hipEventDisableTiming));
+        c.device = dev;
+    }
+    *out = &c;
+    return DROT_OK;
+}
+
+struct DevPtrs {
+    SolverState *st;
+    float *phi1, *phi2, *a, *b, *ahat, *bhat;
+    double *prow, *pcol, *pobj, *pfin;
+    int *bounds;
+};
+
+DevPtrs carve(const Plan &P, void *ws) {
+    char *base = static_cast<char *>(ws);
+    DevPtrs d;
+    d.st = reinterpret_cast<SolverState *>(base + P.off_state);
+    d.phi1 = reinterpret_cast<float *>(base + P.off_phi1);
+    d.phi2 = reinterpret_cast<float *>(base + P.off_phi2);
+    d.a = reinterpret_cast<float *>(base + P.off_a);
+    d.b = reinterpret_cast<float *>(base + P.off_b);
+    d.ahat = reinterpret_cast<float *>(base + P.off_ahat);
+    d.bhat = reinterpret_cast<float *>(base + P.off_bhat);
+    d.prow = reinterpret_cast<double *>(base + P.off_prow);
+    d.pcol = reinterpret_cast<double *>(base + P.off_pcol);
+    d.pobj = reinterpret_cast<double *>(base + P.off_pobj);
+    d.pfin = reinterpret_cast<double *>(base + P.off_pfin);
+    d.bounds = reinterpret_cast<int *>(base + P.off_bounds);
+    return d;
+}
+
+// Warm-start constants, evaluated in fp32 exactly as the reference's host code
+// (pytorch/src/qr_drot_cuda_kernel.cu:53-61, core/drot_qr.hpp:322-330).
+void warm_constants(int n_rows, int n_cols, float step, InitArgs *ia) {
+    const float n = float(n_rows);
+    const float m = float(n_cols);
+    const float k = float(1.0) * step * n * m - float(2.);
+    ia->v_phi1 = (k + float(2)) / n / (m + n);
+    ia->v_phi2 = (k + float(2)) / m / (m + n);
+    ia->v_a = (k + float(1)) / n;
+    ia->v_b = (k + float(1)) / m;
+    ia->v_alpha = k + float(1);
+}
+
+// Iteration chunk: keep each chunk around a millisecond of GPU work so the host
+// stays ahead, and at most 64 iterations so little work is wasted after
+// convergence.  Always even, so the QR even/odd kernels keep their phase.
+int chunk_iters(const Plan &P) {
+    const double elems = (double)P.n_rows * (double)P.n_cols;
+    const double us_per_iter = std::max(4.0, elems * 12.0 / 5.0e6);  // ~5 TB/s
+    int ch = (int)(1000.0 / us_per_iter);
+    ch = std::max(2, std::min(64, ch));
+    return ch & ~1;
+}
+
+struct SolveIn {
+    const float *c, *p, *q;
+    float *x;
+    float step, kappa, lambda;
+    int max_iters;
+    float eps;
+    int warm;
+};
+
+drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
+                drot_result *res) {
+    DevPtrs d = carve(P, ws);
+    PollCtx *pc = nullptr;
+    drot_status ps = poll_ctx(&pc);
+    if (ps != DROT_OK) return ps;
+
+    InitArgs ia{};
+    ia.x = in.x;
+    ia.p = in.p;
+    ia.q = in.q;
+    ia.phi1 = d.phi1;
+    ia.phi2 = d.phi2;
+    ia.a = d.a;
+    ia.b = d.b;
+    ia.ahat = d.ahat;
+    ia.bhat = d.bhat;
+    ia.st = d.st;
+    ia.n_rows = P.n_rows;
+    ia.n_cols = P.n_cols;
+    ia.warm = in.warm;
+    warm_constants(P.n_rows, P.n_cols, in.step, &ia);
+    DROT_HIP_CHECK(launch_init(ia, stream));
+
+    if (P.kind == kGL) {
+        // group row bounds, core/kernel_glr.hpp:79-80 (groups <= 256 rows) and
+        // :185-186 (larger groups), evaluated in fp32 as there
+        const int G = P.n_groups;
+        std::vector<int> hb(G + 1);
+        const bool small = ceil_div(P.n_rows, G) <= 256;
+        const double off = small ? 0.49 : 0.49999;
+        const float per = float(P.n_rows) / float(G);
+        for (int g = 0; g <= G; g++) hb[g] = (int)((double)(per * float(g)) + off);
+        if (hb[0] != 0 || hb[G] != P.n_rows)
+            return fail(DROT_ERR_UNSUPPORTED, "group bounds do not cover all rows");
+        DROT_HIP_CHECK(hipMemcpyAsync(d.bounds, hb.data(), sizeof(int) * (G + 1),
+                                      hipMemcpyHostToDevice, stream));
+        DROT_HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    QrPassArgs qa{};
+    GlPassArgs ga{};
+    if (P.kind == kQR) {
+        qa.x = in.x;
+        qa.c = in.c;
+        qa.phi1 = d.phi1;
+        qa.phi2 = d.phi2;
+        qa.ahat = d.ahat;
+        qa.bhat = d.bhat;
+        qa.st = d.st;
+        qa.prow = d.prow;
+        qa.pcol = d.pcol;
+        qa.pobj = d.pobj;
+        qa.n_rows = P.n_rows;
+        qa.n_cols = P.n_cols;
+        qa.n_rt = P.n_rt;
+        qa.n_ct = P.n_ct;
+        qa.nstep = P.nstep;
+        qa.step = in.step;
+        qa.kappa = in.kappa;
+    } else {
+        ga.x = in.x;
+        ga.c = in.c;
+        ga.phi1 = d.phi1;
+        ga.phi2 = d.phi2;
+        ga.ahat = d.ahat;
+        ga.bhat = d.bhat;
+        ga.st = d.st;
+        ga.prow = d.prow;
+        ga.pcol = d.pcol;
+        ga.pobj = d.pobj;
+        ga.bounds = d.bounds;
+        ga.n_rows = P.n_rows;
+        ga.n_cols = P.n_cols;
+        ga.n_groups = P.n_groups;
+        ga.n_ct = P.n_ct;
+        ga.tc = P.tc;
+        ga.lane_per_group = P.lane_per_group;
+        ga.step = in.step;
+        ga.lambda = in.lambda;
+    }
+    FinalizeArgs fa{};
+    fa.prow = d.prow;
+    fa.pcol = d.pcol;
+    fa.pobj = d.pobj;
+    fa.p = in.p;
+    fa.q = in.q;
+    fa.a = d.a;
+    fa.b = d.b;
+    fa.ahat = d.ahat;
+    fa.bhat = d.bhat;
+    fa.pfin = d.pfin;
+    fa.st = d.st;
+    fa.n_rows = P.n_rows;
+    fa.n_cols = P.n_cols;
+    fa.n_rt = P.n_rt;
+    fa.n_ct = P.n_ct;
+    fa.n_obj = P.n_obj;
+    fa.rb = P.rb;
+    fa.cb = P.cb;
+    fa.max_iters = in.max_iters;
+    fa.eps = in.eps;
+    fa.stop_rule = (P.kind == kQR) ? kStopEvenStrict : kStopEveryLeq;
+    fa.obj_even_only = (P.kind == kQR) ? 1 : 0;
+    const int grid_fin = P.rb + P.cb;
+
+    const int chunk = chunk_iters(P);
+    int launched = 0;
+    int slot = 0;
+    bool pending[2] = {false, false};
+    volatile int *flags = pc->pinned;
+    while (launched < in.max_iters) {
+        const int n = std::min(chunk, in.max_iters - launched);
+        for (int t = 0; t < n; t++) {
+            const int k = launched + t;
+            if (P.kind == kQR) {
+                DROT_HIP_CHECK(launch_qr_pass(P.qt, (k % 2) == 0, P.grid_pass, qa, stream));
+            } else {
+                DROT_HIP_CHECK(launch_gl_pass(P.gt, P.grid_pass, P.lds, ga, stream));
+            }
+            DROT_HIP_CHECK(launch_finalize(grid_fin, fa, stream));
+        }
+        launched += n;
+        DROT_HIP_CHECK(hipMemcpyAsync((void *)&flags[slot], &d.st->done, sizeof(int),
+                                      hipMemcpyDeviceToHost, stream));
+        DROT_HIP_CHECK(hipEventRecord(pc->ev[slot], stream));
+        pending[slot] = true;
+        const int other = slot ^ 1;
+        if (pending[other]) {
+            DROT_HIP_CHECK(hipEventSynchronize(pc->ev[other]));
+            pending[other] = false;
+            if (flags[other]) break;
+        }
+        slot = other;
+    }
+    if (P.kind == kQR) {
+        DROT_HIP_CHECK(launch_fix_x(in.x, in.c, in.step,
+                                    (size_t)P.n_rows * (size_t)P.n_cols, d.st, stream));
+    }
+    SolverState *hs = reinterpret_cast<SolverState *>(pc->pinned + 16);
+    DROT_HIP_CHECK(hipMemcpyAsync(hs, d.st, sizeof(SolverState), hipMemcpyDeviceToHost,
+                                  stream));
+    DROT_HIP_CHECK(hipStreamSynchronize(stream));
+    if (res) {
+        res->n_iter = hs->done ? hs->n_iter : (P.kind == kQR ? in.max_iters + 1 : in.max_iters);
+        res->converged = hs->converged;
+        res->iterations_run = hs->k;
+        res->objective = (float)hs->obj;
+        res->objective_f64 = hs->obj;
+        res->residual = hs->res;
+        res->run_ms = 0.0f;
+        res->prep_ms = 0.0f;
+    }
+    return DROT_OK;
+}
+
+drot_status check_common(const float *c, const float *p, const float *q, int n_rows,
+                         int n_cols, const float *x, int max_iters) {
+    if (!c || !p || !q || !x) return fail(DROT_ERR_INVALID_ARGUMENT, "null pointer argument");
+    if (n_rows <= 0 || n_cols <= 0)
+        return fail(DROT_ERR_INVALID_ARGUMENT, "n_rows and n_cols must be positive");
+    if ((long long)n_rows * (long long)n_cols > (1LL << 40))
+        return fail(DROT_ERR_UNSUPPORTED, "matrix too large");
+    if (max_iters < 0) return fail(DROT_ERR_INVALID_ARGUMENT, "max_iters must be >= 0");
+    return DROT_OK;
+}
+
+}  // namespace
+}  // namespace drot
+
+using namespace drot;
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+int drot_abi_version(void) { return DROT_ABI_VERSION; }
+
+const char *drot_last_error(void) { return g_last_error.c_str(); }
+
+const char *drot_status_string(drot_status s) {
+    switch (s) {
+        case DROT_OK: return "ok";
+        case DROT_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case DROT_ERR_WORKSPACE_TOO_SMALL: return "workspace too small";
+        case DROT_ERR_UNSUPPORTED: return "unsupported shape";
+        case DROT_ERR_HIP: return "HIP runtime error";
+        case DROT_ERR_ALLOC: return "allocation failed";
+    }
+    return "unknown status";
+}
+
+drot_status drot_qr_params_from_rho(float rho, float r_weight, int n_rows, int n_cols,
+                                    int max_iters, float eps, drot_qr_params *out) {
+    if (!out || n_rows <= 0 || n_cols <= 0)
+        return fail(DROT_ERR_INVALID_ARGUMENT, "bad arguments to drot_qr_params_from_rho");
+    // pytorch/src/qr_drot_cuda_kernel.cu:38-40
+    const float step = rho / (float(n_rows) + float(n_cols));
+    out->step_size = step;
+    out->scale = 1 + step * r_weight * float(n_rows + n_cols);
+    out->r_weight = r_weight;
+    out->max_iters = max_iters;
+    out->eps = eps;
+    out->warm_start = 1;
+    return DROT_OK;
+}
+
+drot_status drot_qr_params_from_step(float step_size, float r_weight, int n_rows,
+                                     int n_cols, int max_iters, float eps,
+                                     int warm_start, drot_qr_params *out) {
+    if (!out || n_rows <= 0 || n_cols <= 0)
+        return fail(DROT_ERR_INVALID_ARGUMENT, "bad arguments to drot_qr_params_from_step");
+    // core/drot_qr.hpp:262
+    out->step_size = step_size;
+    out->scale = (1 + step_size * float(n_rows + n_cols) * r_weight);
+    out->r_weight = r_weight;
+    out->max_iters = max_iters;
+    out->eps = eps;
+    out->warm_start = warm_start ? 1 : 0;
+    return DROT_OK;
+}
+
+drot_status drot_glr_params_from_rho(float rho, float r_weight, int n_rows, int n_cols,
+                                     int n_groups, int max_iters, float eps,
+                                     drot_glr_params *out) {
+    if (!out || n_rows <= 0 || n_cols <= 0 || n_groups <= 0)
+        return fail(DROT_ERR_INVALID_ARGUMENT, "bad arguments to drot_glr_params_from_rho");
+    // pytorch/src/glr_drot_cuda_kernel.cu:45-47
+    out->step_size = rho / (float(n_rows) + float(n_cols));
+    out->lambda = (float)((double)r_weight * sqrt((double)(n_rows / n_groups)));
+    out->n_groups = n_groups;
+    out->max_iters = max_iters;
+    out->eps = eps;
+    out->warm_start = 1;
+    return DROT_OK;
+}
+
+drot_status drot_glr_params_from_step(float step_size, float r_weight, int n_rows,
+                                      int n_cols, int n_groups, int max_iters,
+                                      float eps, int warm_start, drot_glr_params *out) {
+    if (!out || n_rows <= 0 || n_cols <= 0 || n_groups <= 0)
+        return fail(DROT_ERR_INVALID_ARGUMENT, "bad arguments to drot_glr_params_from_step");
+    // core/drot_glr.hpp:385
+    out->step_size = step_size;
+    out->lambda = (float)((double)r_weight / sqrt((double)(n_rows / n_groups)));
+    out->n_groups = n_groups;
+    out->max_iters = max_iters;
+    out->eps = eps;
+    out->warm_start = warm_start ? 1 : 0;
+    return DROT_OK;
+}
+
+drot_status drot_qr_workspace_size(int n_rows, int n_cols, size_t *bytes) {
+    if (!bytes || n_rows <= 0 || n_cols <= 0)
+        return fail(DROT_ERR_INVALID_ARGUMENT, "bad arguments to drot_qr_workspace_size");
+    const Plan a = plan_qr(n_rows, n_cols, true);
+    const Plan b = plan_qr(n_rows, n_cols, false);
+    *bytes = std::max(a.total, b.total);
+    return DROT_OK;
+}
+
+drot_status drot_glr_workspace_size(int n_rows, int n_cols, int n_groups, size_t *bytes) {
+    if (!bytes || n_rows <= 0 || n_cols <= 0 || n_groups <= 0)
+        return fail(DROT_ERR_INVALID_ARGUMENT, "bad arguments to drot_glr_workspace_size");
+    Plan a, b;
+    std::string why;
+    if (!plan_gl(n_rows, n_cols, n_groups, true, &a, &why)) return fail(DROT_ERR_UNSUPPORTED, why);
+    if (!plan_gl(n_rows, n_cols, n_groups, false, &b, &why)) return fail(DROT_ERR_UNSUPPORTED, why);
+    *bytes = std::max(a.total, b.total);
+    return DROT_OK;
+}
+
+drot_status drot_qr_solve_f32(const float *c, const float *p, const float *q, int n_rows,
+                              int n_cols, const drot_qr_params *prm, float *x,
+                              void *workspace, size_t workspace_bytes, void *stream,
+                              drot_result *result) {
+    drot_status s = check_common(c, p, q, n_rows, n_cols, x, prm ? prm->max_iters : 0);
+    if (s != DROT_OK) return s;
+    if (!prm) return fail(DROT_ERR_INVALID_ARGUMENT, "params is NULL");
+    if (!workspace) return fail(DROT_ERR_INVALID_ARGUMENT, "workspace is NULL");
+    if (!(prm->scale > 0.0f)) return fail(DROT_ERR_INVALID_ARGUMENT, "scale must be > 0");
+    const Plan P = plan_qr(n_rows, n_cols, aligned16(c) && aligned16(x));
+    if (workspace_bytes < P.total)
+        return fail(DROT_ERR_WORKSPACE_TOO_SMALL,
+                    "workspace needs " + std::to_string(P.total) + " bytes");
+    SolveIn in{c, p, q, x, prm->step_size, prm->scale, 0.0f, prm->max_iters, prm->eps,
+               prm->warm_start};
+    return run(P, in, workspace, static_cast<hipStream_t>(stream), result);
+}
+
+drot_status drot_glr_solve_f32(const float *c, const float *p, const float *q, int n_rows,
+                               int n_cols, const drot_glr_params *prm, float *x,
+                               void *workspace, size_t workspace_bytes, void *stream,
+                               drot_result *result) {
+    drot_status s = check_common(c, p, q, n_rows, n_cols, x, prm ? prm->max_iters : 0);
+    if (s != DROT_OK) return s;
+    if (!prm) return fail(DROT_ERR_INVALID_ARGUMENT, "params is NULL");
+    if (!workspace) return fail(DROT_ERR_INVALID_ARGUMENT, "workspace is NULL");
+    if (prm->n_groups <= 0 || prm->n_groups > n_rows)
+        return fail(DROT_ERR_INVALID_ARGUMENT, "n_groups must be in [1, n_rows]");
+    Plan P;
+    std::string why;
+    if (!plan_gl(n_rows, n_cols, prm->n_groups, aligned16(c) && aligned16(x), &P, &why))
+        return fail(DROT_ERR_UNSUPPORTED, why);
+    if (workspace_bytes < P.total)
+        return fail(DROT_ERR_WORKSPACE_TOO_SMALL,
+                    "workspace needs " + std::to_string(P.total) + " bytes");
+    SolveIn in{c, p, q, x, prm->step_size, 1.0f, prm->lambda, prm->max_iters, prm->eps,
+               prm->warm_start};
+    return run(P, in, workspace, static_cast<hipStream_t>(stream), result);
+}
+
+drot_status drot_export_duals_f32(int kind, int n_rows, int n_cols, int n_groups,
+                                  const void *workspace, size_t workspace_bytes,
+                                  float *phi1, float *phi2, float *alpha, void *stream) {
+    if (!workspace || n_rows <= 0 || n_cols <= 0)
+        return fail(DROT_ERR_INVALID_ARGUMENT, "bad arguments to drot_export_duals_f32");
+    // Offsets of the dual vectors and state do not depend on the tile choice
+    // (they precede the partial buffers), so either plan's layout serves.
+    Plan P;
+    std::string why;
+    if (kind == kQR) {
+        P = plan_qr(n_rows, n_cols, true);
+    } else if (kind == kGL) {
+        if (!plan_gl(n_rows, n_cols, n_groups, true, &P, &why)) return fail(DROT_ERR_UNSUPPORTED, why);
+    } else {
+        return fail(DROT_ERR_INVALID_ARGUMENT, "kind must be 0 (QR) or 1 (GL)");
+    }
+    if (workspace_bytes < P.off_prow)
+        return fail(DROT_ERR_WORKSPACE_TOO_SMALL, "workspace too small for this shape");
+    DevPtrs d = carve(P, const_cast<void *>(workspace));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (phi1 || phi2) {
+        float *o1 = phi1, *o2 = phi2;
+        if (!o1 || !o2)
+            return fail(DROT_ERR_INVALID_ARGUMENT, "phi1 and phi2 must both be given");
+        DROT_HIP_CHECK(launch_export_phi(d.ahat, d.bhat, d.phi1, d.phi2, d.st, n_rows,
+                                         n_cols, o1, o2, s));
+    }
+    if (alpha) {
+        SolverState hs;
+        DROT_HIP_CHECK(hipMemcpyAsync(&hs, d.st, sizeof(hs), hipMemcpyDeviceToHost, s));
+        DROT_HIP_CHECK(hipStreamSynchronize(s));
+        *alpha = hs.alpha;
+    } else {
+        DROT_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return DROT_OK;
+}
+
+void drot_free(void *ptr) { free(ptr); }
+
+}  // extern "C"
+
+// ----------------------------------------------------------------- wrappers
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+// Shared body of the two host-pointer wrappers: allocate, copy, solve, copy
+// back, free -- with the prep / run split of core/drot_qr.hpp:271-366.
+template <typename SolveFn>
+float *host_wrapper(const float *hc, const float *hp, const float *hq, int n_rows,
+                    int n_cols, size_t ws_bytes, float *dur_in_ms, float *prep_dur_in_ms,
+                    int *n_iter, float *objective, int return_x, SolveFn solve) {
+    if (!hc || !hp || !hq || n_rows <= 0 || n_cols <= 0) {
+        fail(DROT_ERR_INVALID_ARGUMENT, "bad arguments to wrapper");
+        return nullptr;
+    }
+    const size_t mat = (size_t)n_rows * (size_t)n_cols * sizeof(float);
+    DevBuf c, p, q, x, ws;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    float *out = nullptr;
+    auto ok = [&](hipError_t e, const char *what) {
+        if (e != hipSuccess) {
+            fail(DROT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+            return false;
+        }
+        return true;
+    };
+    if (!ok(hipEventCreate(&e0), "hipEventCreate") || !ok(hipEventCreate(&e1), "hipEventCreate") ||
+        !ok(hipEventCreate(&e2), "hipEventCreate"))
+        goto done;
+    if (!ok(hipEventRecord(e0, nullptr), "hipEventRecord")) goto done;
+    if (!ok(hipMalloc(&c.p, mat), "hipMalloc") || !ok(hipMalloc(&x.p, mat), "hipMalloc") ||
+        !ok(hipMalloc(&p.p, (size_t)n_rows * 4), "hipMalloc") ||
+        !ok(hipMalloc(&q.p, (size_t)n_cols * 4), "hipMalloc") ||
+        !ok(hipMalloc(&ws.p, ws_bytes), "hipMalloc"))
+        goto done;
+    if (!ok(hipMemcpy(c.p, hc, mat, hipMemcpyHostToDevice), "hipMemcpy") ||
+        !ok(hipMemcpy(p.p, hp, (size_t)n_rows * 4, hipMemcpyHostToDevice), "hipMemcpy") ||
+        !ok(hipMemcpy(q.p, hq, (size_t)n_cols * 4, hipMemcpyHostToDevice), "hipMemcpy"))
+        goto done;
+    if (!ok(hipEventRecord(e1, nullptr), "hipEventRecord")) goto done;
+    {
+        drot_result r{};
+        const drot_status s = solve((const float *)c.p, (const float *)p.p, (const float *)q.p,
+                                    (float *)x.p, ws.p, ws_bytes, &r);
+        if (s != DROT_OK) goto done;
+        if (!ok(hipEventRecord(e2, nullptr), "hipEventRecord") ||
+            !ok(hipEventSynchronize(e2), "hipEventSynchronize"))
+            goto done;
+        float prep = 0.0f, run_ms = 0.0f;
+        (void)hipEventElapsedTime(&prep, e0, e1);
+        (void)hipEventElapsedTime(&run_ms, e1, e2);
+        if (dur_in_ms) *dur_in_ms = run_ms;
+        if (prep_dur_in_ms) *prep_dur_in_ms = prep;
+        if (n_iter) *n_iter = r.n_iter;
+        if (objective) *objective = r.objective;
+        if (return_x) {
+            out = (float *)malloc(mat);
+            if (!out) {
+                fail(DROT_ERR_ALLOC, "malloc of the returned plan failed");
+                goto done;
+            }
+            if (!ok(hipMemcpy(out, x.p, mat, hipMemcpyDeviceToHost), "hipMemcpy")) {
+                free(out);
+                out = nullptr;
+                goto done;
+            }
+        }
+        g_last_error.clear();
+    }
+done:
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e2) (void)hipEventDestroy(e2);
+    return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+float *drot_qr_wrapper_f32(const float *c, const float *p, const float *q, int n_rows,
+                           int n_cols, float step_size, float r_weight, int max_iters,
+                           float eps, float *dur_in_ms, float *prep_dur_in_ms, int *n_iter,
+                           float *objective, int use_warmup_init, int return_x) {
+    drot_qr_params prm;
+    if (drot_qr_params_from_step(step_size, r_weight, n_rows, n_cols, max_iters, eps,
+                                 use_warmup_init, &prm) != DROT_OK)
+        return nullptr;
+    size_t ws = 0;
+    if (drot_qr_workspace_size(n_rows, n_cols, &ws) != DROT_OK) return nullptr;
+    return host_wrapper(c, p, q, n_rows, n_cols, ws, dur_in_ms, prep_dur_in_ms, n_iter,
+                        objective, return_x,
+                        [&](const float *dc, const float *dp, const float *dq, float *dx,
+                            void *dws, size_t wsb, drot_result *r) {
+                            return drot_qr_solve_f32(dc, dp, dq, n_rows, n_cols, &prm, dx,
+                                                     dws, wsb, nullptr, r);
+                        });
+}
+
+float *drot_glr_wrapper_f32(const float *c, const float *p, const float *q, int n_rows,
+                            int n_cols, int n_groups, float step_size, float r_weight,
+                            int max_iters, float eps, float *dur_in_ms,
+                            float *prep_dur_in_ms, int *n_iter, float *objective,
+                            int use_warmup_init, int return_x) {
+    drot_glr_params prm;
+    if (drot_glr_params_from_step(step_size, r_weight, n_rows, n_cols, n_groups, max_iters,
+                                  eps, use_warmup_init, &prm) != DROT_OK)
+        return nullptr;
+    size_t ws = 0;
+    if (drot_glr_workspace_size(n_rows, n_cols, n_groups, &ws) != DROT_OK) return nullptr;
+    return host_wrapper(c, p, q, n_rows, n_cols, ws, dur_in_ms, prep_dur_in_ms, n_iter,
+                        objective, return_x,
+                        [&](const float *dc, const float *dp, const float *dq, float *dx,
+                            void *dws, size_t wsb, drot_result *r) {
+                            return drot_glr_solve_f32(dc, dp, dq, n_rows, n_cols, &prm, dx,
+                                                      dws, wsb, nullptr, r);
+                        });
+}
+
+}  // extern "C"
