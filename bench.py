@@ -1,0 +1,207 @@
+"""Benchmark of the QR-DROT hot path on one MI355X.
+
+Workload (one "step" = one full solve from the warm start to primal residual
+1e-4, inputs resident in HBM): QR-DROT 3000 x 3000 fp32, C = squared
+Euclidean distances between two iid N(0, I_16) point clouds divided by max(C),
+uniform marginals, r_weight (alpha) = 1e-3, rho = 2.0, tol 1e-4.
+
+value = algorithmic HBM throughput of the whole solve in GB/s:
+    bytes = sum over executed iterations of 12*m*n (even k: read C and X,
+            write X) or 8*m*n (odd k: read X, write X -- the even/odd -sC
+            folding of core/kernel_qr.hpp is kept), divided by wall time.
+The roofline block times the fused pass kernel itself with HIP events on the
+solver's stream (library timing mode) over the timed steps.  cpu_baseline is
+the CPU oracle (a C port of the same iteration, OpenMP over all host threads)
+timed on a bounded sample of iterations of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "regularized-drot_amd"))
+sys.path.insert(0, REPO)
+
+N_ROWS = N_COLS = 3000
+R_WEIGHT = 1e-3
+RHO = 2.0
+EPS = 1e-4
+MAX_ITERS = 20000
+HBM_PEAK_GBS = 8000.0
+METRIC = ("time-to-1e-4 + HBM GB/s, QR 3000² fp32; batched fwd+bwd solves/s "
+          "vs looped plain")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-sample-s", type=float, default=15.0,
+                    help="target seconds of CPU-baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def solve_bytes(iters: int, n: int) -> float:
+    n_even = (iters + 1) // 2
+    n_odd = iters // 2
+    return float(n_even * 12 * n + n_odd * 8 * n)
+
+
+def traffic_from_profiles():
+    """HBM bytes per pass launch from the committed rocprofv3 PMC summary."""
+    path = os.path.join(REPO, "profiles", "qr_pass_3000x3000_pmc.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as fh:
+            return json.load(fh).get("traffic_bytes_per_even_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(c, p, q, step, kappa, target_s):
+    import numpy as np
+    import oracle as O
+
+    threads = O.default_threads()
+    st = O.warm_start(N_ROWS, N_COLS, step)
+    n = N_ROWS * N_COLS
+    # calibrate on 2 iterations, then run a bounded even number of iterations
+    t0 = time.perf_counter()
+    for k in range(2):
+        O.qr_step(st, k, c, p, q, step, kappa, nthreads=threads)
+    per_pair = max(time.perf_counter() - t0, 1e-3)
+    pairs = max(1, int(target_s / per_pair))
+    st = O.warm_start(N_ROWS, N_COLS, step)
+    t0 = time.perf_counter()
+    for k in range(2 * pairs):
+        O.qr_step(st, k, c, p, q, step, kappa, nthreads=threads)
+    dt = time.perf_counter() - t0
+    gbs = solve_bytes(2 * pairs, n) / dt / 1e9
+    del np
+    return {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"{2 * pairs} QR-DROT iterations of the 3000x3000 workload from the "
+                      f"warm start ({dt:.1f} s, oracle/drot_oracle.c fp32, OpenMP)",
+            "us_per_iter": round(dt / (2 * pairs) * 1e6, 1)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(one OT problem does not shard here; batched problems would shard "
+              "whole problems across GPUs).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from reg_drot import _abi, problems
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    c_np = problems.cloud_cost(N_ROWS, N_COLS, dim=16, seed=0)
+    p_np, q_np = problems.uniform_marginals(N_ROWS, N_COLS)
+    c = torch.from_numpy(c_np).to(dev)
+    p = torch.from_numpy(p_np).to(dev)
+    q = torch.from_numpy(q_np).to(dev)
+    x = torch.empty_like(c)
+    wsb = _abi.qr_workspace_size(N_ROWS, N_COLS)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    prm = _abi.qr_params_from_rho(RHO, R_WEIGHT, N_ROWS, N_COLS, MAX_ITERS, EPS)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def run_once():
+        return _abi.qr_solve(c.data_ptr(), p.data_ptr(), q.data_ptr(), N_ROWS, N_COLS, prm,
+                             x.data_ptr(), ws.data_ptr(), wsb, stream)
+
+    for _ in range(args.warmup):
+        run_once()
+    torch.cuda.synchronize(dev)
+
+    # timed steps (plain mode)
+    results = []
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        results.append(run_once())
+    torch.cuda.synchronize(dev)
+    wall = time.perf_counter() - t0
+    iters = [r.iterations_run for r in results]
+    assert all(r.converged for r in results), "solve did not converge"
+    total_bytes = sum(solve_bytes(it, N_ROWS * N_COLS) for it in iters)
+    gbs = total_bytes / wall / 1e9
+    ms_per_step = wall / args.steps * 1e3
+
+    # pass-kernel roofline: same steps again in timing mode (events on the stream)
+    _abi.set_timing(True)
+    even_ms = odd_ms = 0.0
+    even_n = odd_n = 0
+    for _ in range(max(1, args.steps)):
+        r = run_once()
+        even_ms += r.pass_ms[0]
+        odd_ms += r.pass_ms[1]
+        even_n += r.pass_count[0]
+        odd_n += r.pass_count[1]
+    _abi.set_timing(False)
+    n = N_ROWS * N_COLS
+    even_avg_us = even_ms / max(even_n, 1) * 1e3
+    odd_avg_us = odd_ms / max(odd_n, 1) * 1e3
+    achieved = 12.0 * n / (even_avg_us * 1e-6) / 1e9  # even pass: the dominant kernel
+    odd_gbs = 8.0 * n / (odd_avg_us * 1e-6) / 1e9
+
+    last = results[-1]
+    out = {
+        "metric": METRIC,
+        "value": round(gbs, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "QR-DROT 3000x3000 fp32 solve to primal residual 1e-4 "
+                               "(warm start, rho=2, r_weight=1e-3, C=sqdist/max of "
+                               "N(0,I_16) clouds, uniform marginals)",
+                   "n_rows": N_ROWS, "n_cols": N_COLS, "r_weight": R_WEIGHT, "rho": RHO,
+                   "eps": EPS},
+        "time_to_tol_ms": round(ms_per_step, 4),
+        "iterations": last.iterations_run,
+        "n_iter_reference_convention": last.n_iter,
+        "us_per_iter": round(ms_per_step * 1e3 / max(last.iterations_run, 1), 3),
+        "residual": last.residual,
+        "objective": last.objective_f64,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(),
+            "kernel": "qr_pass_kernel<4,32,EVEN> (12 B/element: read C, read X, write X)",
+            "avg_us_even": round(even_avg_us, 3),
+            "avg_us_odd": round(odd_avg_us, 3),
+            "odd_pass_GBs": round(odd_gbs, 1),
+        },
+    }
+    if not args.no_cpu:
+        s = np.float32(prm.step_size)
+        k = np.float32(prm.scale)
+        out["cpu_baseline"] = cpu_baseline(c_np, p_np, q_np, s, k, args.cpu_sample_s)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

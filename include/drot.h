@@ -72,6 +72,10 @@ typedef struct drot_result {
                               the last iteration that ran the stopping test    */
     float run_ms;          /* wrappers only: solve time (as dur_in_ms)         */
     float prep_ms;         /* wrappers only: alloc + copies + init             */
+    double pass_ms[2];     /* timing mode only (drot_set_timing): summed device
+                              time of the fused pass kernels that did work,
+                              [0] = QR even iterations / GL, [1] = QR odd     */
+    int pass_count[2];     /* number of passes summed in pass_ms               */
 } drot_result;
 
 /* ---------------------------------------------------------------- params */
@@ -165,6 +169,11 @@ const char *drot_status_string(drot_status status);
 /* Message of the last failing call on this thread ("" if none). */
 const char *drot_last_error(void);
 int drot_abi_version(void);
+
+/* Timing mode for the calling thread: when enabled, solves bracket every fused
+ * pass launch with HIP events on the solve's stream and report the per-pass
+ * device time in drot_result.pass_ms.  Off by default. */
+void drot_set_timing(int enable);
 
 #ifdef __cplusplus
 }

@@ -200,6 +200,30 @@ drot_status poll_ctx(PollCtx **out) {
     return DROT_OK;
 }
 
+thread_local bool g_timing = false;
+
+// Per-thread pool of timing events (timing mode only).
+struct TimingPool {
+    std::vector<hipEvent_t> ev;
+    int device = -1;
+};
+thread_local TimingPool g_tpool;
+
+drot_status timing_pool(int pairs) {
+    int dev = 0;
+    DROT_HIP_CHECK(hipGetDevice(&dev));
+    if (g_tpool.device != dev) {
+        g_tpool.ev.clear();  // events of another device are kept alive
+        g_tpool.device = dev;
+    }
+    while ((int)g_tpool.ev.size() < 2 * pairs) {
+        hipEvent_t e = nullptr;
+        DROT_HIP_CHECK(hipEventCreate(&e));
+        g_tpool.ev.push_back(e);
+    }
+    return DROT_OK;
+}
+
 struct DevPtrs {
     SolverState *st;
     float *phi1, *phi2, *a, *b, *ahat, *bhat;
@@ -365,19 +389,47 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
     const int grid_fin = P.rb + P.cb;
 
     const int chunk = chunk_iters(P);
+    // Timing mode: events around every pass; pool halves alternate between the
+    // (at most two) chunks in flight, and a half is read back before reuse.
+    const bool timing = g_timing;
+    std::vector<float> pass_ms;
+    if (timing) {
+        drot_status ts = timing_pool(2 * chunk);
+        if (ts != DROT_OK) return ts;
+        pass_ms.assign((size_t)std::max(in.max_iters, 1), 0.0f);
+    }
+    int chunk_start[2] = {0, 0}, chunk_n[2] = {0, 0};
+    auto harvest = [&](int half) -> drot_status {
+        for (int t = 0; t < chunk_n[half]; t++) {
+            hipEvent_t e0 = g_tpool.ev[(size_t)(half * chunk + t) * 2];
+            hipEvent_t e1 = g_tpool.ev[(size_t)(half * chunk + t) * 2 + 1];
+            float ms = 0.0f;
+            DROT_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            pass_ms[(size_t)(chunk_start[half] + t)] = ms;
+        }
+        chunk_n[half] = 0;
+        return DROT_OK;
+    };
     int launched = 0;
     int slot = 0;
     bool pending[2] = {false, false};
     volatile int *flags = pc->pinned;
     while (launched < in.max_iters) {
         const int n = std::min(chunk, in.max_iters - launched);
+        chunk_start[slot] = launched;
+        chunk_n[slot] = timing ? n : 0;
         for (int t = 0; t < n; t++) {
             const int k = launched + t;
+            if (timing)
+                DROT_HIP_CHECK(hipEventRecord(g_tpool.ev[(size_t)(slot * chunk + t) * 2], stream));
             if (P.kind == kQR) {
                 DROT_HIP_CHECK(launch_qr_pass(P.qt, (k % 2) == 0, P.grid_pass, qa, stream));
             } else {
                 DROT_HIP_CHECK(launch_gl_pass(P.gt, P.grid_pass, P.lds, ga, stream));
             }
+            if (timing)
+                DROT_HIP_CHECK(
+                    hipEventRecord(g_tpool.ev[(size_t)(slot * chunk + t) * 2 + 1], stream));
             DROT_HIP_CHECK(launch_finalize(grid_fin, fa, stream));
         }
         launched += n;
@@ -389,6 +441,10 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
         if (pending[other]) {
             DROT_HIP_CHECK(hipEventSynchronize(pc->ev[other]));
             pending[other] = false;
+            if (timing) {
+                drot_status hs = harvest(other);
+                if (hs != DROT_OK) return hs;
+            }
             if (flags[other]) break;
         }
         slot = other;
@@ -401,6 +457,12 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
     DROT_HIP_CHECK(hipMemcpyAsync(hs, d.st, sizeof(SolverState), hipMemcpyDeviceToHost,
                                   stream));
     DROT_HIP_CHECK(hipStreamSynchronize(stream));
+    if (timing) {
+        for (int h = 0; h < 2; h++) {
+            drot_status st = harvest(h);
+            if (st != DROT_OK) return st;
+        }
+    }
     if (res) {
         res->n_iter = hs->done ? hs->n_iter : (P.kind == kQR ? in.max_iters + 1 : in.max_iters);
         res->converged = hs->converged;
@@ -410,6 +472,15 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
         res->residual = hs->res;
         res->run_ms = 0.0f;
         res->prep_ms = 0.0f;
+        res->pass_ms[0] = res->pass_ms[1] = 0.0;
+        res->pass_count[0] = res->pass_count[1] = 0;
+        if (timing) {
+            for (int k = 0; k < hs->k && k < (int)pass_ms.size(); k++) {
+                const int par = (P.kind == kQR && (k % 2) == 1) ? 1 : 0;
+                res->pass_ms[par] += pass_ms[(size_t)k];
+                res->pass_count[par] += 1;
+            }
+        }
     }
     return DROT_OK;
 }
@@ -435,6 +506,8 @@ using namespace drot;
 extern "C" {
 
 int drot_abi_version(void) { return DROT_ABI_VERSION; }
+
+void drot_set_timing(int enable) { g_timing = enable != 0; }
 
 const char *drot_last_error(void) { return g_last_error.c_str(); }
 

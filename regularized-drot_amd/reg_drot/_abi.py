@@ -25,7 +25,7 @@ EXPORTS = (
     "drot_qr_workspace_size", "drot_glr_workspace_size",
     "drot_qr_solve_f32", "drot_glr_solve_f32", "drot_export_duals_f32",
     "drot_qr_wrapper_f32", "drot_glr_wrapper_f32", "drot_free",
-    "drot_status_string", "drot_last_error", "drot_abi_version",
+    "drot_status_string", "drot_last_error", "drot_abi_version", "drot_set_timing",
 )
 
 DROT_OK = 0
@@ -47,10 +47,14 @@ class Result(ctypes.Structure):
     _fields_ = [("n_iter", ctypes.c_int), ("converged", ctypes.c_int),
                 ("iterations_run", ctypes.c_int), ("objective", ctypes.c_float),
                 ("objective_f64", ctypes.c_double), ("residual", ctypes.c_double),
-                ("run_ms", ctypes.c_float), ("prep_ms", ctypes.c_float)]
+                ("run_ms", ctypes.c_float), ("prep_ms", ctypes.c_float),
+                ("pass_ms", ctypes.c_double * 2), ("pass_count", ctypes.c_int * 2)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["pass_ms"] = list(self.pass_ms)
+        d["pass_count"] = list(self.pass_count)
+        return d
 
 
 class DrotError(RuntimeError):
@@ -101,6 +105,8 @@ def load(path: str = LIB_PATH):
     lib.drot_last_error.restype = ctypes.c_char_p
     lib.drot_abi_version.argtypes = []
     lib.drot_abi_version.restype = c_int
+    lib.drot_set_timing.argtypes = [c_int]
+    lib.drot_set_timing.restype = None
     for name in ("drot_qr_params_from_rho", "drot_qr_params_from_step",
                  "drot_glr_params_from_rho", "drot_glr_params_from_step",
                  "drot_qr_workspace_size", "drot_glr_workspace_size",
@@ -174,6 +180,10 @@ def glr_solve(c, p, q, n_rows, n_cols, prm: GlrParams, x, ws, ws_bytes, stream) 
     check(load().drot_glr_solve_f32(c, p, q, n_rows, n_cols, ctypes.byref(prm), x, ws,
                                     ws_bytes, stream, ctypes.byref(r)), "drot_glr_solve_f32")
     return r
+
+
+def set_timing(enable: bool) -> None:
+    load().drot_set_timing(int(bool(enable)))
 
 
 def export_duals(kind, n_rows, n_cols, n_groups, ws, ws_bytes, phi1, phi2, stream):

@@ -1,0 +1,106 @@
+"""CPU-side checks of the C ABI (no GPU calls): libdrot.so loads, exports every
+function include/drot.h declares, its parameter helpers reproduce the
+reference's conventions bit for bit (same fp32 expressions as the oracle), and
+argument errors are reported as status codes before any device work."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "drot.h")
+
+
+def lib():
+    from reg_drot import _abi
+    return _abi.load()
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(drot_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_expected_api():
+    from reg_drot import _abi
+    assert declared_functions() == sorted(_abi.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = lib()
+    for name in declared_functions():
+        assert hasattr(L, name), name
+    assert L.drot_abi_version() == 1
+
+
+def test_no_cpu_fallback_symbols():
+    """The product library holds no oracle code."""
+    from reg_drot import _abi
+    out = os.popen(f"nm -D --defined-only {_abi.LIB_PATH}").read()
+    assert "oracle_" not in out
+
+
+@pytest.mark.parametrize("n_rows,n_cols,rho,rw", [(3000, 3000, 2.0, 1e-3), (1000, 5000, 2.0, 1e-5),
+                                                  (7, 3, 1.5, 10.0), (2, 3, 2.0, 1e-2)])
+def test_qr_params_match_reference_conventions(n_rows, n_cols, rho, rw):
+    from reg_drot import _abi
+    prm = _abi.qr_params_from_rho(rho, rw, n_rows, n_cols, 100, 1e-4)
+    s, k = O.qr_params_torch(rho, rw, n_rows, n_cols)
+    assert np.float32(prm.step_size) == s and np.float32(prm.scale) == k
+    assert prm.warm_start == 1
+    core = _abi.qr_params_from_step(float(s), rw, n_rows, n_cols, 100, 1e-4, warm=False)
+    assert np.float32(core.scale) == O.qr_kappa_core(s, rw, n_rows, n_cols)
+    assert core.warm_start == 0
+
+
+@pytest.mark.parametrize("n_rows,n_cols,g,rw", [(1500, 1000, 2, 1e-3), (5000, 5000, 10, 1e-5),
+                                                (10, 20, 2, 1e-3), (7, 3, 7, 0.5)])
+def test_glr_params_match_reference_conventions(n_rows, n_cols, g, rw):
+    from reg_drot import _abi
+    prm = _abi.glr_params_from_rho(2.0, rw, n_rows, n_cols, g, 100, 1e-4)
+    assert np.float32(prm.lambda_) == O.gl_lambda_torch(rw, n_rows, g)
+    assert np.float32(prm.step_size) == O.step_torch(2.0, n_rows, n_cols)
+    core = _abi.glr_params_from_step(float(prm.step_size), rw, n_rows, n_cols, g, 100, 1e-4)
+    assert np.float32(core.lambda_) == O.gl_lambda_core(rw, n_rows, g)
+
+
+def test_workspace_sizes_are_modest():
+    from reg_drot import _abi
+    n = 3000
+    ws = _abi.qr_workspace_size(n, n)
+    assert 0 < ws < 0.1 * n * n * 4  # partials are a few percent of X
+    assert _abi.glr_workspace_size(1500, 1000, 2) > 0
+    with pytest.raises(_abi.DrotError, match="unsupported"):
+        _abi.glr_workspace_size(20000, 10, 2)
+
+
+def test_argument_errors_are_status_codes():
+    from reg_drot import _abi
+    L = lib()
+    prm = _abi.qr_params_from_rho(2.0, 1e-3, 4, 4, 10, 1e-4)
+    r = _abi.Result()
+    st = L.drot_qr_solve_f32(None, None, None, 4, 4, ctypes.byref(prm), None, None, 0, None,
+                             ctypes.byref(r))
+    assert st == 1 and b"null" in L.drot_last_error()
+    st = L.drot_qr_workspace_size(0, 4, ctypes.byref(ctypes.c_size_t()))
+    assert st == 1
+    gp = _abi.glr_params_from_rho(2.0, 1e-3, 4, 4, 2, 10, 1e-4)
+    gp.n_groups = 9
+    fake = ctypes.c_void_p(16)  # never dereferenced: validation fails first
+    st = L.drot_glr_solve_f32(fake, fake, fake, 4, 4, ctypes.byref(gp), fake, fake, 1 << 20,
+                              None, ctypes.byref(r))
+    assert st == 1 and b"n_groups" in L.drot_last_error()
+    prm.scale = 0.0
+    st = L.drot_qr_solve_f32(fake, fake, fake, 4, 4, ctypes.byref(prm), fake, fake, 1 << 20,
+                             None, ctypes.byref(r))
+    assert st == 1
+    prm = _abi.qr_params_from_rho(2.0, 1e-3, 4, 4, 10, 1e-4)
+    st = L.drot_qr_solve_f32(fake, fake, fake, 4, 4, ctypes.byref(prm), fake, fake, 8,
+                             None, ctypes.byref(r))
+    assert st == 2  # workspace too small
+    assert L.drot_status_string(3) == b"unsupported shape"
