@@ -46,6 +46,12 @@ def parse():
     ap.add_argument("--cpu-sample-s", type=float, default=15.0,
                     help="target seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-iters", type=int, default=0,
+                    help="run one solve of exactly this many iterations and exit "
+                         "(for rocprofv3 counter runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run plain device copies of one 3000x3000 fp32 matrix and exit "
+                         "(reference reading for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -108,6 +114,15 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        a = torch.empty(N_ROWS * N_COLS, dtype=torch.float32, device=dev).uniform_()
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize(dev)
+        print(json.dumps({"copy_calibration_bytes_read": 4 * N_ROWS * N_COLS,
+                          "copy_calibration_bytes_written": 4 * N_ROWS * N_COLS, "copies": 5}))
+        return
     c_np = problems.cloud_cost(N_ROWS, N_COLS, dim=16, seed=0)
     p_np, q_np = problems.uniform_marginals(N_ROWS, N_COLS)
     c = torch.from_numpy(c_np).to(dev)
@@ -122,6 +137,14 @@ def main():
     def run_once():
         return _abi.qr_solve(c.data_ptr(), p.data_ptr(), q.data_ptr(), N_ROWS, N_COLS, prm,
                              x.data_ptr(), ws.data_ptr(), wsb, stream)
+
+    if args.profile_iters > 0:
+        pprm = _abi.qr_params_from_rho(RHO, R_WEIGHT, N_ROWS, N_COLS, args.profile_iters, 0.0)
+        r = _abi.qr_solve(c.data_ptr(), p.data_ptr(), q.data_ptr(), N_ROWS, N_COLS, pprm,
+                          x.data_ptr(), ws.data_ptr(), wsb, stream)
+        torch.cuda.synchronize(dev)
+        print(json.dumps({"profile_iters": r.iterations_run, "residual": r.residual}))
+        return
 
     for _ in range(args.warmup):
         run_once()
@@ -190,7 +213,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic_from_profiles(),
-            "kernel": "qr_pass_kernel<4,32,EVEN> (12 B/element: read C, read X, write X)",
+            "kernel": "qr_pass_kernel<VEC=4,LPC=32,EVEN,ST=sc1,U=2> (12 B/element: read C, "
+                      "read X, write X)",
             "avg_us_even": round(even_avg_us, 3),
             "avg_us_odd": round(odd_avg_us, 3),
             "odd_pass_GBs": round(odd_gbs, 1),

@@ -26,27 +26,38 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 // Solver state kept in device memory for the whole solve.  The host never
 // reads it inside the iteration loop except through asynchronous snapshots.
 struct alignas(64) SolverState {
-    int k;            // index of the next iteration to run
-    int done;         // 1 once stopped (converged or max_iters exhausted)
-    int converged;    // 1 if the stopping test passed
-    int phi_mode;     // 1: phi1/phi2 arrays are explicit; 0: phi = (hat + gamma)/n
-    float gamma;      // shift of the last dual update (core/kernel_qr.hpp:421)
-    float alpha;      // aux[4] of the reference (core/drot_qr.hpp:74-76)
-    int n_iter;       // iteration count in the reference's convention
-    unsigned int counter;  // arrival counter of the finalize kernel
-    double obj;       // objective of the last iteration that computed it
-    double res;       // primal residual of the last checked iteration
-    double xsum;      // sum of X' of the last iteration
-    double res_last;  // primal residual of the last iteration (any parity)
-    double obj_last;  // objective of the last iteration (0 on QR odd iterations)
-    int last_k;       // index of the last executed iteration (-1 before any)
-    int pad_[3];
+    int done;             // 1 once stopped (converged or max_iters exhausted)
+    int converged;        // 1 if the stopping test passed
+    int n_iter;           // iteration count in the reference's convention
+    int k_run;            // iterations executed
+    float gamma;          // gamma of the last dual update (core/kernel_qr.hpp:421)
+    float alpha;          // alpha after the last dual update (aux[4], core/drot_qr.hpp:74-76)
+    float alpha_slot[2];  // alpha after iteration j lives in slot j % 2 (slot 1: initial)
+    int phi_mode;         // 1 until the first dual update: phi arrays are explicit
+    int last_k;           // index of the last finished iteration (-1 before any)
+    double obj;           // objective of the last iteration that computed it
+    double res;           // primal residual of the last iteration that was tested
+    double xsum;          // sum of X' of the last iteration
+    double res_last;      // primal residual of the last iteration (any parity)
 };
 
-// Which stopping rule the finalize kernel applies.
+// Which stopping rule the solve applies.
 enum StopRule : int {
     kStopEvenStrict = 0,  // QR: test on even k only, residual < eps (core/drot_qr.hpp:217-233)
     kStopEveryLeq = 1,    // GL: test every k, residual <= eps (core/drot_glr.hpp:344-354)
+};
+
+// Per-launch iteration control shared by the pass kernels.
+struct IterCtl {
+    SolverState *st;
+    const double *pfin;  // [n_fin][3]: residual part, sum-X part, objective part
+    int n_fin;
+    int k;               // iteration this launch runs
+    int max_iters;
+    float eps;
+    int stop_rule;       // StopRule
+    int obj_even_only;   // QR: objective only on even iterations
+    int n_rows, n_cols;
 };
 
 // ---------------------------------------------------------------- reductions
@@ -76,8 +87,104 @@ __device__ __forceinline__ double block_sum(double v, double *scratch) {
     return t;
 }
 
+// Fixed-order block-wide sums of three doubles at once (one LDS round trip).
+// `scratch` holds 3 * kWavesPerBlock doubles.  Results valid in every thread.
+__device__ __forceinline__ void block_sum3(double &a, double &b, double &c, double *scratch) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        scratch[w] = a;
+        scratch[kWavesPerBlock + w] = b;
+        scratch[2 * kWavesPerBlock + w] = c;
+    }
+    __syncthreads();
+    double ta = 0.0, tb = 0.0, tc = 0.0;
+#pragma unroll
+    for (int i = 0; i < kWavesPerBlock; i++) {
+        ta += scratch[i];
+        tb += scratch[kWavesPerBlock + i];
+        tc += scratch[2 * kWavesPerBlock + i];
+    }
+    __syncthreads();
+    a = ta;
+    b = tb;
+    c = tc;
+}
+
 __device__ __forceinline__ int wave_uniform(int v) {
     return __builtin_amdgcn_readfirstlane(v);
+}
+
+struct IterStart {
+    bool finish;        // the solve ended with iteration k-1: do not touch X
+    bool explicit_phi;  // k == 0: phi1/phi2 arrays hold the initial phi
+    float gamma;        // gamma of iteration k-1: phi = (hat + gamma) / n
+};
+
+// Prologue of the pass of iteration k, run by every thread of every block;
+// s_red holds 3 * kWavesPerBlock doubles.
+// Finishes iteration k-1 from the per-block partials of its finalize launch,
+// identically in every block (same fixed-order sums):
+//   S = sum X', residual = sqrt(sum rs^2 + sum cs^2), objective,
+//   gamma = (2 (S - 1) - alpha) / (n_rows + n_cols)   (alpha of iteration k-2)
+//   alpha <- (alpha + 1) - S                          (core/kernel_qr.hpp:421,463-467)
+// and applies the stopping test of iteration k-1 (core/drot_qr.hpp:217-229 or
+// core/drot_glr.hpp:344-354).  Block 0 records the result.  Because the test
+// runs before any block writes X, a converged iterate is never overwritten.
+__device__ inline IterStart iteration_prologue(const IterCtl &C, double *s_red) {
+    SolverState *st = C.st;
+    IterStart r;
+    r.explicit_phi = true;
+    r.gamma = 0.0f;
+    const int k = C.k;
+    const bool leader = (blockIdx.x == 0) && (threadIdx.x == 0);
+    bool conv = false;
+    bool finish = k >= C.max_iters;
+    if (k > 0) {
+        double rr = 0.0, ss = 0.0, oo = 0.0;
+        for (int t = threadIdx.x; t < C.n_fin; t += kBlock) {
+            rr += C.pfin[3 * t];
+            ss += C.pfin[3 * t + 1];
+            oo += C.pfin[3 * t + 2];
+        }
+        block_sum3(rr, ss, oo, s_red);
+        const double S = ss;
+        const double obj = oo;
+        const double res = sqrt(rr);
+        const float alpha_prev = st->alpha_slot[k & 1];  // after iteration k-2
+        const float gamma =
+            (float)((2.0 * (S - 1.0) - (double)alpha_prev) / (double)(C.n_rows + C.n_cols));
+        const float alpha_new = (float)(((double)alpha_prev + 1.0) - S);
+        const bool prev_even = ((k - 1) & 1) == 0;
+        const bool every = C.stop_rule == kStopEveryLeq;
+        const bool check = every || prev_even;
+        const bool has_obj = !C.obj_even_only || prev_even;
+        conv = check && (every ? (res <= (double)C.eps) : (res < (double)C.eps));
+        finish = finish || conv;
+        if (leader) {
+            st->alpha_slot[(k - 1) & 1] = alpha_new;
+            st->alpha = alpha_new;
+            st->gamma = gamma;
+            st->phi_mode = 0;
+            st->xsum = S;
+            st->res_last = res;
+            if (check) st->res = res;
+            if (has_obj) st->obj = obj;
+            st->last_k = k - 1;
+        }
+        r.explicit_phi = false;
+        r.gamma = gamma;
+    }
+    if (finish && leader) {
+        st->converged = conv ? 1 : 0;
+        st->n_iter = conv ? k : (C.stop_rule == kStopEveryLeq ? C.max_iters : C.max_iters + 1);
+        st->k_run = k;
+        st->done = 1;
+    }
+    r.finish = finish;
+    return r;
 }
 
 }  // namespace drot

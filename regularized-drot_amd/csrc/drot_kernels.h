@@ -8,22 +8,20 @@
 
 namespace drot {
 
-// Arguments of the fused QR pass (one DR iteration over X).
+// Arguments of the fused QR pass (drot_qr.hip): one DR iteration over X.
 struct QrPassArgs {
     float *x;
     const float *c;
-    const float *phi1;  // explicit phi (SolverState::phi_mode == 1)
-    const float *phi2;
-    const float *ahat;  // phi numerators (phi_mode == 0): phi1 = (ahat + gamma) / n_cols
-    const float *bhat;  //                                 phi2 = (bhat + gamma) / n_rows
-    const SolverState *st;
-    double *prow;       // [n_ct][n_rows] row partial sums
-    double *pcol;       // [n_rt][n_cols] column partial sums
-    double *pobj;       // [n_rt * n_ct] objective partials
+    const float *phi1, *phi2;  // explicit phi (SolverState::phi_mode == 1)
+    const float *ahat, *bhat;  // phi numerators: phi1 = (ahat + gamma)/n_cols, ...
+    double *prow;              // [n_ct][n_rows] row partial sums
+    double *pcol;              // [n_rt][n_cols] column partial sums
+    double *pobj;              // [n_rt * n_ct] objective partials (even k)
     int n_rows, n_cols;
-    int n_rt, n_ct;     // row tiles, column tiles
-    int nstep;          // column steps per wave; tile columns = nstep * 4 * (64 / lpc)
+    int n_rt, n_ct;            // row tiles, column tiles
+    int nstep;                 // column steps per wave; tile columns = nstep * 4 * (64 / lpc)
     float step, kappa;
+    IterCtl ctl;
 };
 
 // Arguments of the fused GL pass (prox + group soft-threshold + sums).
@@ -31,7 +29,6 @@ struct GlPassArgs {
     float *x;
     const float *c;
     const float *phi1, *phi2, *ahat, *bhat;
-    const SolverState *st;
     double *prow;       // [n_ct][n_rows]
     double *pcol;       // [n_cols]
     double *pobj;       // [n_ct]
@@ -40,6 +37,7 @@ struct GlPassArgs {
     int n_ct, tc;       // column tiles, columns per tile
     int lane_per_group; // 1: one lane sums a group serially (small groups)
     float step, lambda;
+    IterCtl ctl;
 };
 
 // Arguments of the dual / aux update (shared by QR and GL).
@@ -47,15 +45,12 @@ struct FinalizeArgs {
     const double *prow, *pcol, *pobj;
     const float *p, *q;
     float *a, *b, *ahat, *bhat;
-    double *pfin;       // [rb + cb][2]
-    SolverState *st;
+    double *pfin;       // [rb + cb][3]: residual part, sum-X part, objective part
+    const SolverState *st;
     int n_rows, n_cols, n_rt, n_ct;
     int n_obj;          // number of objective partials
-    int rb, cb;         // row blocks, column blocks
-    int max_iters;
-    float eps;
-    int stop_rule;      // StopRule
-    int obj_even_only;  // 1 for QR (objective on even iterations only)
+    int rb, cb;         // row blocks, column blocks (kFinRows each)
+    int with_obj;       // this iteration produced objective partials
 };
 
 struct InitArgs {
@@ -69,8 +64,10 @@ struct InitArgs {
 };
 
 struct QrTile {
-    int vec;   // 1 or 4 floats per lane access
-    int lpc;   // lanes per column: 16, 32 or 64
+    int vec;     // 1 or 4 floats per lane access
+    int lpc;     // lanes per column: 16, 32 or 64
+    int nt = 0;  // X store policy: 0 default (L2 write-back), 2 write-through (sc1)
+    int u = 4;   // column steps per load batch (2 or 4)
 };
 
 struct GlTile {
@@ -92,6 +89,9 @@ hipError_t launch_export_phi(const float *ahat, const float *bhat, const float *
                              hipStream_t s);
 
 bool qr_tile_supported(const QrTile &t);
+
+// rows (or columns) reduced per finalize block
+constexpr int kFinRows = 64;
 bool gl_tile_supported(const GlTile &t);
 
 }  // namespace drot

@@ -77,141 +77,23 @@ __global__ __launch_bounds__(kBlock) void init_kernel(InitArgs A) {
     }
     if (tid == 0) {
         SolverState s;
-        s.k = 0;
         s.done = 0;
         s.converged = 0;
-        s.phi_mode = 1;
+        s.n_iter = 0;
+        s.k_run = 0;
         s.gamma = 0.0f;
         s.alpha = A.warm ? A.v_alpha : 0.0f;
-        s.n_iter = 0;
-        s.counter = 0u;
+        s.alpha_slot[0] = 0.0f;
+        s.alpha_slot[1] = s.alpha;  // "after iteration -1"
+        s.phi_mode = 1;
+        s.last_k = -1;
         s.obj = 0.0;
         s.res = 1e9;
         s.xsum = 0.0;
         s.res_last = 1e9;
-        s.obj_last = 0.0;
-        s.last_k = -1;
-        s.pad_[0] = s.pad_[1] = s.pad_[2] = 0;
         *A.st = s;
     }
-}
 
-// ----------------------------------------------------------------- QR pass
-
-// Fused QR-DROT pass, restating quadratic_regularizer_update_x_{even,odd}
-// (core/kernel_qr.hpp:171-376) for one tile of TR = LPC*VEC rows by
-// nstep*4*(64/LPC) columns:
-//   X' = max(X + phi1 + phi2 - s C, 0) / kappa      (EVEN; ODD drops the -sC,
-//                                                    which is already in X)
-//   stored X = X' - s C (EVEN) | X' (ODD)
-// Lane layout: LPC lanes cover VEC*LPC contiguous rows of one column, so a wave
-// handles 64/LPC columns at once and every access is a coalesced 16-byte load.
-// The column (contiguous-axis) partial is a butterfly over those LPC lanes; the
-// row (strided-axis) partials live in per-lane fp64 registers across the tile's
-// columns and are combined through LDS in a fixed order at the end.
-template <int VEC, int LPC, bool EVEN>
-__global__ __launch_bounds__(kBlock) void qr_pass_kernel(QrPassArgs A) {
-    constexpr int CPW = kWave / LPC;  // columns per wave step
-    constexpr int TR = LPC * VEC;     // tile rows
-    constexpr int U = 4;              // column steps in flight per lane
-    __shared__ double s_row[kWavesPerBlock * CPW][TR];
-    __shared__ double s_red[kWavesPerBlock];
-
-    const SolverState *st = A.st;
-    if (st->done) return;
-    const int phi_mode = st->phi_mode;
-    const float gamma = st->gamma;
-
-    const int N = A.n_rows, M = A.n_cols;
-    const int rt = blockIdx.x % A.n_rt;
-    const int ct = blockIdx.x / A.n_rt;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int w = wave_uniform(threadIdx.x / kWave);
-    const int cs = lane / LPC;
-    const int rl = lane % LPC;
-    const int row = rt * TR + rl * VEC;
-    const bool row_ok = row < N;  // VEC == 4 only when N % 4 == 0
-    const int tc = A.nstep * kWavesPerBlock * CPW;
-    const int col0 = ct * tc;
-    const float fM = (float)M, fN = (float)N;
-    const float s = A.step, kappa = A.kappa;
-
-    float f1[VEC];
-#pragma unroll
-    for (int e = 0; e < VEC; e++)
-        f1[e] = row_ok ? phi_of(phi_mode, A.phi1, A.ahat, row + e, gamma, fM) : 0.0f;
-
-    double racc[VEC];
-#pragma unroll
-    for (int e = 0; e < VEC; e++) racc[e] = 0.0;
-    double oacc = 0.0;
-
-    for (int t0 = 0; t0 < A.nstep; t0 += U) {
-        float xv[U][VEC], cv[U][VEC];
-        int jj[U];
-        bool ok[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int t = t0 + u;
-            jj[u] = col0 + (t * kWavesPerBlock + w) * CPW + cs;
-            ok[u] = (t < A.nstep) && (jj[u] < M) && row_ok;
-            if (ok[u]) {
-                const size_t off = (size_t)jj[u] * (size_t)N + (size_t)row;
-                VecIO<VEC>::load(A.x + off, xv[u]);
-                if (EVEN) VecIO<VEC>::load(A.c + off, cv[u]);
-            } else {
-#pragma unroll
-                for (int e = 0; e < VEC; e++) { xv[u][e] = 0.0f; cv[u][e] = 0.0f; }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int t = t0 + u;
-            const int j = jj[u];
-            const bool col_ok = (t < A.nstep) && (j < M);
-            const float f2 = col_ok ? phi_of(phi_mode, A.phi2, A.bhat, j, gamma, fN) : 0.0f;
-            float xo[VEC];
-            double csum = 0.0;
-#pragma unroll
-            for (int e = 0; e < VEC; e++) {
-                const float tt = (xv[u][e] + f1[e]) + f2;
-                float y = EVEN ? __builtin_fmaf(-s, cv[u][e], tt) : tt;
-                y = fmaxf(y, 0.0f);
-                const float xp = y / kappa;
-                xo[e] = EVEN ? __builtin_fmaf(-s, cv[u][e], xp) : xp;
-                if (ok[u]) {
-                    racc[e] += (double)xp;
-                    csum += (double)xp;
-                    if (EVEN) oacc += (double)xp * (double)cv[u][e];
-                }
-            }
-            if (ok[u]) {
-                const size_t off = (size_t)j * (size_t)N + (size_t)row;
-                VecIO<VEC>::store(A.x + off, xo);
-            }
-            if (t0 + u < A.nstep) {  // wave-uniform
-                csum = seg_sum<LPC>(csum);
-                if (rl == 0 && j < M) A.pcol[(size_t)rt * M + j] = csum;
-            }
-        }
-    }
-
-    // strided-axis partials: fixed-order combine of the 4*CPW column slots
-    const int slot = w * CPW + cs;
-#pragma unroll
-    for (int e = 0; e < VEC; e++) s_row[slot][rl * VEC + e] = racc[e];
-    __syncthreads();
-    for (int r = threadIdx.x; r < TR; r += kBlock) {
-        double sum = 0.0;
-#pragma unroll
-        for (int k = 0; k < kWavesPerBlock * CPW; k++) sum += s_row[k][r];
-        const int i = rt * TR + r;
-        if (i < N) A.prow[(size_t)ct * N + i] = sum;
-    }
-    if (EVEN) {
-        const double o = block_sum(oacc, s_red);
-        if (threadIdx.x == 0) A.pobj[blockIdx.x] = o;
-    }
 }
 
 // ----------------------------------------------------------------- GL pass
@@ -229,11 +111,12 @@ __global__ __launch_bounds__(kBlock) void qr_pass_kernel(QrPassArgs A) {
 template <int VEC, int RCH>
 __global__ __launch_bounds__(kBlock) void gl_pass_kernel(GlPassArgs A) {
     extern __shared__ double s_dyn[];
-    __shared__ double s_red[kWavesPerBlock];
-    const SolverState *st = A.st;
-    if (st->done) return;
-    const int phi_mode = st->phi_mode;
-    const float gamma = st->gamma;
+    __shared__ double s_red[3 * kWavesPerBlock];
+    if (A.ctl.st->done) return;
+    const IterStart is = iteration_prologue(A.ctl, s_red);
+    if (is.finish) return;
+    const int phi_mode = is.explicit_phi ? 1 : 0;
+    const float gamma = is.gamma;
     const int N = A.n_rows, M = A.n_cols, G = A.n_groups;
     double *s_sq = s_dyn;                                   // [N]
     float *s_scale = reinterpret_cast<float *>(s_dyn + N);  // [G]
@@ -352,122 +235,83 @@ __global__ __launch_bounds__(kBlock) void gl_pass_kernel(GlPassArgs A) {
 
 // ----------------------------------------------------------------- finalize
 
-// Dual / aux update shared by QR and GL: quadratic_regularizer_update_aux +
-// update_alpha + the residual reductions (core/kernel_qr.hpp:405-467,
-// core/drot_qr.hpp:145-156; identical in core/kernel_glr.hpp:461-526):
+// Dual update of iteration k, shared by QR and GL: the row and column halves
+// of quadratic_regularizer_update_aux (core/kernel_qr.hpp:405-461, identical
+// in core/kernel_glr.hpp:461-520) plus the residual reductions of
+// core/drot_qr.hpp:145-156:
 //   rowsum = sum_ct prow ; rs = rowsum - p ; a <- a - rs ; ahat = a - rs
 //   colsum = sum_rt pcol ; cs = colsum - q ; b <- b - cs ; bhat = b - cs
-// Blocks [0, rb) own rows, [rb, rb+cb) own columns.  The last block to arrive
-// forms S = sum X', gamma = (2(S-1) - alpha)/(n_rows+n_cols) (old alpha),
-// alpha <- (alpha+1) - S, the residual and the stopping test.  phi itself is
-// formed by the next pass as (hat + gamma)/n, so no second grid-wide step is
-// needed here.
+// Blocks [0, rb) own kFinRows rows each, [rb, rb+cb) kFinRows columns; four
+// threads share a row so all partial loads of a row are in flight at once.
+// Each block leaves {sum rs^2 or cs^2, sum X', objective slice} in pfin; the
+// scalar part (gamma, alpha, residual, stopping test) is formed from those by
+// the prologue of the next pass, so this kernel has no grid-wide step and no
+// agent-scope fence.
 __global__ __launch_bounds__(kBlock) void finalize_kernel(FinalizeArgs A) {
-    __shared__ double s_red[kWavesPerBlock];
-    __shared__ int s_last;
-    SolverState *st = A.st;
-    if (st->done) return;
+    constexpr int kParts = kBlock / kFinRows;
+    __shared__ double s_part[kParts][kFinRows];
+    __shared__ double s_red[3 * kWavesPerBlock];
+    if (A.st->done) return;
     const int N = A.n_rows, M = A.n_cols;
     const int b = blockIdx.x;
-    double res = 0.0, S = 0.0;
-    if (b < A.rb) {
-        const int i = b * kBlock + threadIdx.x;
-        if (i < N) {
-            const double *src = A.prow + i;
-            double sum = 0.0;
-            int ct = 0;
-            for (; ct + 8 <= A.n_ct; ct += 8) {
-                double v[8];
+    const int part = threadIdx.x / kFinRows;
+    const int li = threadIdx.x % kFinRows;
+    const bool is_row = b < A.rb;
+    const int idx = (is_row ? b : b - A.rb) * kFinRows + li;
+    const int len = is_row ? N : M;
+    const int cnt = is_row ? A.n_ct : A.n_rt;
+    const bool valid = idx < len;
+    const double *src = (is_row ? A.prow : A.pcol) + (valid ? idx : 0);
+    double sum = 0.0;
+    if (valid) {
+        int t = part;
+        for (; t + (8 - 1) * kParts < cnt; t += 8 * kParts) {
+            double v[8];
 #pragma unroll
-                for (int u = 0; u < 8; u++) v[u] = src[(size_t)(ct + u) * N];
+            for (int u = 0; u < 8; u++) v[u] = src[(size_t)(t + u * kParts) * len];
 #pragma unroll
-                for (int u = 0; u < 8; u++) sum += v[u];
-            }
-            for (; ct < A.n_ct; ct++) sum += src[(size_t)ct * N];
-            const double d = sum - (double)A.p[i];
-            const float rs = (float)d;
-            const float an = A.a[i] - rs;
-            A.a[i] = an;
-            A.ahat[i] = an - rs;
-            res = d * d;
+            for (int u = 0; u < 8; u++) sum += v[u];
         }
-    } else {
-        const int j = (b - A.rb) * kBlock + threadIdx.x;
-        if (j < M) {
-            const double *src = A.pcol + j;
-            double sum = 0.0;
-            int rt = 0;
-            for (; rt + 8 <= A.n_rt; rt += 8) {
-                double v[8];
-#pragma unroll
-                for (int u = 0; u < 8; u++) v[u] = src[(size_t)(rt + u) * M];
-#pragma unroll
-                for (int u = 0; u < 8; u++) sum += v[u];
-            }
-            for (; rt < A.n_rt; rt++) sum += src[(size_t)rt * M];
-            const double d = sum - (double)A.q[j];
-            const float cs = (float)d;
-            const float bn = A.b[j] - cs;
-            A.b[j] = bn;
-            A.bhat[j] = bn - cs;
-            res = d * d;
-            S = sum;
-        }
+        for (; t < cnt; t += kParts) sum += src[(size_t)t * len];
     }
-    res = block_sum(res, s_red);
-    S = block_sum(S, s_red);
-    if (threadIdx.x == 0) {
-        A.pfin[2 * b] = res;
-        A.pfin[2 * b + 1] = S;
-        __threadfence();
-        const unsigned int prev = atomicAdd(&st->counter, 1u);
-        s_last = (prev == gridDim.x - 1) ? 1 : 0;
-    }
+    s_part[part][li] = sum;
     __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-
-    const int k = st->k;
-    const bool even = (k % 2) == 0;
-    const bool has_obj = !A.obj_even_only || even;
-    double rr = 0.0, rc = 0.0, SS = 0.0, obj = 0.0;
-    for (int bb = threadIdx.x; bb < A.rb; bb += kBlock) rr += A.pfin[2 * bb];
-    for (int bb = A.rb + threadIdx.x; bb < A.rb + A.cb; bb += kBlock) {
-        rc += A.pfin[2 * bb];
-        SS += A.pfin[2 * bb + 1];
-    }
-    if (has_obj)
-        for (int bb = threadIdx.x; bb < A.n_obj; bb += kBlock) obj += A.pobj[bb];
-    rr = block_sum(rr, s_red);
-    rc = block_sum(rc, s_red);
-    SS = block_sum(SS, s_red);
-    obj = block_sum(obj, s_red);
-    if (threadIdx.x == 0) {
-        const float alpha = st->alpha;
-        st->gamma = (float)((2.0 * (SS - 1.0) - (double)alpha) / (double)(N + M));
-        st->alpha = (float)(((double)alpha + 1.0) - SS);
-        st->phi_mode = 0;
-        st->xsum = SS;
-        const double resid = sqrt(rr + rc);
-        st->res_last = resid;
-        st->obj_last = has_obj ? obj : 0.0;
-        st->last_k = k;
-        const bool check = (A.stop_rule == kStopEveryLeq) || even;
-        if (check) st->res = resid;
-        if (has_obj) st->obj = obj;
-        const bool conv = check && ((A.stop_rule == kStopEveryLeq)
-                                        ? (resid <= (double)A.eps)
-                                        : (resid < (double)A.eps));
-        if (conv) {
-            st->converged = 1;
-            st->done = 1;
-            st->n_iter = k + 1;
-        } else if (k + 1 >= A.max_iters) {
-            st->done = 1;
-            st->n_iter = (A.stop_rule == kStopEveryLeq) ? A.max_iters : A.max_iters + 1;
+    double res = 0.0, S = 0.0;
+    if (part == 0 && valid) {
+        double tot = 0.0;
+#pragma unroll
+        for (int q = 0; q < kParts; q++) tot += s_part[q][li];
+        if (is_row) {
+            const double d = tot - (double)A.p[idx];
+            const float rs = (float)d;
+            const float an = A.a[idx] - rs;
+            A.a[idx] = an;
+            A.ahat[idx] = an - rs;
+            res = d * d;
+        } else {
+            const double d = tot - (double)A.q[idx];
+            const float cs = (float)d;
+            const float bn = A.b[idx] - cs;
+            A.b[idx] = bn;
+            A.bhat[idx] = bn - cs;
+            res = d * d;
+            S = tot;
         }
-        st->k = k + 1;
-        st->counter = 0u;
+    }
+    // this block's slice of the pass's objective partials
+    double obj = 0.0;
+    if (A.with_obj) {
+        const int n_blk = A.rb + A.cb;
+        const int per = (A.n_obj + n_blk - 1) / n_blk;
+        const int o0 = b * per;
+        const int o1 = min(o0 + per, A.n_obj);
+        for (int t = o0 + threadIdx.x; t < o1; t += kBlock) obj += A.pobj[t];
+    }
+    block_sum3(res, S, obj, s_red);
+    if (threadIdx.x == 0) {
+        A.pfin[3 * b] = res;
+        A.pfin[3 * b + 1] = S;
+        A.pfin[3 * b + 2] = obj;
     }
 }
 
@@ -477,6 +321,7 @@ __global__ __launch_bounds__(kBlock) void finalize_kernel(FinalizeArgs A) {
 // (convergence is only tested on even iterations, whose stored X is X' - sC).
 __global__ __launch_bounds__(kBlock) void fix_x_kernel(float *x, const float *c, float step,
                                                       size_t total, const SolverState *st) {
+    // fix_x runs once after the loop; st->converged is final by then
     if (!st->converged) return;
     const size_t nthr = (size_t)gridDim.x * blockDim.x;
     const size_t n4 = total / 4;
@@ -523,32 +368,6 @@ hipError_t launch_init(const InitArgs &a, hipStream_t s) {
     const size_t total = (size_t)a.n_rows * (size_t)a.n_cols;
     const int grid = grid_for(total, kBlock * 8, 8192);
     hipLaunchKernelGGL(init_kernel, dim3(grid), dim3(kBlock), 0, s, a);
-    return hipGetLastError();
-}
-
-bool qr_tile_supported(const QrTile &t) {
-    return (t.vec == 1 || t.vec == 4) && (t.lpc == 16 || t.lpc == 32 || t.lpc == 64);
-}
-
-template <int VEC, int LPC>
-static void launch_qr_t(bool even, int grid, const QrPassArgs &a, hipStream_t s) {
-    if (even)
-        hipLaunchKernelGGL((qr_pass_kernel<VEC, LPC, true>), dim3(grid), dim3(kBlock), 0, s, a);
-    else
-        hipLaunchKernelGGL((qr_pass_kernel<VEC, LPC, false>), dim3(grid), dim3(kBlock), 0, s, a);
-}
-
-hipError_t launch_qr_pass(const QrTile &t, bool even, int grid, const QrPassArgs &a,
-                          hipStream_t s) {
-    if (t.vec == 4) {
-        if (t.lpc == 16) launch_qr_t<4, 16>(even, grid, a, s);
-        else if (t.lpc == 32) launch_qr_t<4, 32>(even, grid, a, s);
-        else launch_qr_t<4, 64>(even, grid, a, s);
-    } else {
-        if (t.lpc == 16) launch_qr_t<1, 16>(even, grid, a, s);
-        else if (t.lpc == 32) launch_qr_t<1, 32>(even, grid, a, s);
-        else launch_qr_t<1, 64>(even, grid, a, s);
-    }
     return hipGetLastError();
 }
 

@@ -63,13 +63,14 @@ struct Plan {
     size_t off_pfin = 0, off_bounds = 0, total = 0;
 };
 
-// Optional override of the QR tile, "vec,lpc,nstep" (tuning sweeps only).
+// Optional override of the QR tile, "vec,lpc,nstep[,nt[,u]]" (tuning sweeps only).
 bool qr_tile_from_env(int vec_allowed, QrTile *t, int *nstep) {
     const char *s = getenv("DROT_QR_TILE");
     if (!s || !*s) return false;
-    int v = 0, l = 0, n = 0;
-    if (sscanf(s, "%d,%d,%d", &v, &l, &n) != 3) return false;
-    QrTile cand{v, l};
+    int v = 0, l = 0, n = 0, nt = 0, u = 4;
+    const int got = sscanf(s, "%d,%d,%d,%d,%d", &v, &l, &n, &nt, &u);
+    if (got < 3) return false;
+    QrTile cand{v, l, got >= 4 ? nt : 0, got >= 5 ? u : 4};
     if (!qr_tile_supported(cand) || n < 1) return false;
     if (v == 4 && !vec_allowed) return false;
     *t = cand;
@@ -95,13 +96,16 @@ void layout(Plan &P) {
     P.off_prow = take((size_t)P.n_ct * N * 8);
     P.off_pcol = take((size_t)P.n_rt * M * 8);
     P.off_pobj = take((size_t)P.n_obj * 8);
-    P.off_pfin = take((size_t)(P.rb + P.cb) * 16);
+    P.off_pfin = take((size_t)(P.rb + P.cb) * 24);
     P.off_bounds = take((size_t)(P.n_groups + 1) * 4);
     P.total = off;
 }
 
-// QR tile choice.  The pass wants ~1000 resident blocks (256 CUs, 4 waves per
-// block) and few strided-axis partials (one fp64 per row per column tile).
+// QR tile choice (measured on MI355X at 3000^2 and 10000^2): 16-byte accesses,
+// 32 lanes per column for tall matrices, write-through X stores, two batches
+// of two column steps in flight, about 4096 elements per block, and at least
+// ~1024 blocks when the matrix allows (256 CUs, several blocks resident each).
+// The strided-axis partials cost 8 bytes per row per column tile.
 Plan plan_qr(int n_rows, int n_cols, bool vec4_ok) {
     Plan P;
     P.kind = kQR;
@@ -110,26 +114,24 @@ Plan plan_qr(int n_rows, int n_cols, bool vec4_ok) {
     const bool v4 = vec4_ok && (n_rows % 4 == 0);
     int nstep = 0;
     if (!qr_tile_from_env(v4, &P.qt, &nstep)) {
-        if (v4) P.qt = QrTile{4, n_rows >= 1024 ? 32 : 16};
-        else P.qt = QrTile{1, n_rows >= 256 ? 64 : (n_rows >= 32 ? 32 : 16)};
+        if (v4) P.qt = QrTile{4, n_rows >= 1024 ? 32 : 16, 2, 2};
+        else P.qt = QrTile{1, n_rows >= 256 ? 64 : (n_rows >= 32 ? 32 : 16), 2, 2};
     }
     const int tr = P.qt.lpc * P.qt.vec;
     const int cpw = kWave / P.qt.lpc;
     const int tc_min = kWavesPerBlock * cpw;
     P.n_rt = ceil_div(n_rows, tr);
     if (nstep <= 0) {
-        const int target_blocks = 1024;
-        const int n_ct_target = std::max(1, target_blocks / P.n_rt);
-        const int cols_per = ceil_div(n_cols, n_ct_target);
-        nstep = std::max(1, ceil_div(cols_per, tc_min));
+        nstep = std::max(1, 4096 / (tr * tc_min));
+        while (nstep > 1 && (long long)P.n_rt * ceil_div(n_cols, nstep * tc_min) < 1024) nstep--;
     }
     P.nstep = nstep;
     P.tc = nstep * tc_min;
     P.n_ct = ceil_div(n_cols, P.tc);
     P.grid_pass = P.n_rt * P.n_ct;
     P.n_obj = P.grid_pass;
-    P.rb = ceil_div(n_rows, kBlock);
-    P.cb = ceil_div(n_cols, kBlock);
+    P.rb = ceil_div(n_rows, kFinRows);
+    P.cb = ceil_div(n_cols, kFinRows);
     layout(P);
     return P;
 }
@@ -167,8 +169,8 @@ bool plan_gl(int n_rows, int n_cols, int n_groups, bool vec4_ok, Plan *out,
     P.grid_pass = P.n_ct;
     P.n_obj = P.n_ct;
     P.lane_per_group = (n_rows / std::max(1, n_groups)) < 64 ? 1 : 0;
-    P.rb = ceil_div(n_rows, kBlock);
-    P.cb = ceil_div(n_cols, kBlock);
+    P.rb = ceil_div(n_rows, kFinRows);
+    P.cb = ceil_div(n_cols, kFinRows);
     layout(P);
     *out = P;
     return true;
@@ -306,7 +308,41 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
     warm_constants(P.n_rows, P.n_cols, in.step, &ia);
     DROT_HIP_CHECK(launch_init(ia, stream));
 
-    if (P.kind == kGL) {
+    IterCtl ctl{};
+    ctl.st = d.st;
+    ctl.pfin = d.pfin;
+    ctl.n_fin = P.rb + P.cb;
+    ctl.k = 0;
+    ctl.max_iters = in.max_iters;
+    ctl.eps = in.eps;
+    ctl.stop_rule = (P.kind == kQR) ? kStopEvenStrict : kStopEveryLeq;
+    ctl.obj_even_only = (P.kind == kQR) ? 1 : 0;
+    ctl.n_rows = P.n_rows;
+    ctl.n_cols = P.n_cols;
+
+    QrPassArgs qa{};
+    GlPassArgs ga{};
+    FinalizeArgs fa{};
+    const int grid_fin = P.rb + P.cb;
+    if (P.kind == kQR) {
+        qa.x = in.x;
+        qa.c = in.c;
+        qa.phi1 = d.phi1;
+        qa.phi2 = d.phi2;
+        qa.ahat = d.ahat;
+        qa.bhat = d.bhat;
+        qa.prow = d.prow;
+        qa.pcol = d.pcol;
+        qa.pobj = d.pobj;
+        qa.n_rows = P.n_rows;
+        qa.n_cols = P.n_cols;
+        qa.n_rt = P.n_rt;
+        qa.n_ct = P.n_ct;
+        qa.nstep = P.nstep;
+        qa.step = in.step;
+        qa.kappa = in.kappa;
+        qa.ctl = ctl;
+    } else {
         // group row bounds, core/kernel_glr.hpp:79-80 (groups <= 256 rows) and
         // :185-186 (larger groups), evaluated in fp32 as there
         const int G = P.n_groups;
@@ -320,36 +356,12 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
         DROT_HIP_CHECK(hipMemcpyAsync(d.bounds, hb.data(), sizeof(int) * (G + 1),
                                       hipMemcpyHostToDevice, stream));
         DROT_HIP_CHECK(hipStreamSynchronize(stream));
-    }
-
-    QrPassArgs qa{};
-    GlPassArgs ga{};
-    if (P.kind == kQR) {
-        qa.x = in.x;
-        qa.c = in.c;
-        qa.phi1 = d.phi1;
-        qa.phi2 = d.phi2;
-        qa.ahat = d.ahat;
-        qa.bhat = d.bhat;
-        qa.st = d.st;
-        qa.prow = d.prow;
-        qa.pcol = d.pcol;
-        qa.pobj = d.pobj;
-        qa.n_rows = P.n_rows;
-        qa.n_cols = P.n_cols;
-        qa.n_rt = P.n_rt;
-        qa.n_ct = P.n_ct;
-        qa.nstep = P.nstep;
-        qa.step = in.step;
-        qa.kappa = in.kappa;
-    } else {
         ga.x = in.x;
         ga.c = in.c;
         ga.phi1 = d.phi1;
         ga.phi2 = d.phi2;
         ga.ahat = d.ahat;
         ga.bhat = d.bhat;
-        ga.st = d.st;
         ga.prow = d.prow;
         ga.pcol = d.pcol;
         ga.pobj = d.pobj;
@@ -362,8 +374,8 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
         ga.lane_per_group = P.lane_per_group;
         ga.step = in.step;
         ga.lambda = in.lambda;
+        ga.ctl = ctl;
     }
-    FinalizeArgs fa{};
     fa.prow = d.prow;
     fa.pcol = d.pcol;
     fa.pobj = d.pobj;
@@ -382,12 +394,10 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
     fa.n_obj = P.n_obj;
     fa.rb = P.rb;
     fa.cb = P.cb;
-    fa.max_iters = in.max_iters;
-    fa.eps = in.eps;
-    fa.stop_rule = (P.kind == kQR) ? kStopEvenStrict : kStopEveryLeq;
-    fa.obj_even_only = (P.kind == kQR) ? 1 : 0;
-    const int grid_fin = P.rb + P.cb;
 
+    // launch k (< max_iters): pass + finalize of iteration k; the last launch
+    // (k == max_iters) is a pass whose prologue only finishes iteration k-1
+    const int n_launch = in.max_iters + 1;
     const int chunk = chunk_iters(P);
     // Timing mode: events around every pass; pool halves alternate between the
     // (at most two) chunks in flight, and a half is read back before reuse.
@@ -396,7 +406,7 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
     if (timing) {
         drot_status ts = timing_pool(2 * chunk);
         if (ts != DROT_OK) return ts;
-        pass_ms.assign((size_t)std::max(in.max_iters, 1), 0.0f);
+        pass_ms.assign((size_t)std::max(n_launch, 1), 0.0f);
     }
     int chunk_start[2] = {0, 0}, chunk_n[2] = {0, 0};
     auto harvest = [&](int half) -> drot_status {
@@ -414,8 +424,8 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
     int slot = 0;
     bool pending[2] = {false, false};
     volatile int *flags = pc->pinned;
-    while (launched < in.max_iters) {
-        const int n = std::min(chunk, in.max_iters - launched);
+    while (launched < n_launch) {
+        const int n = std::min(chunk, n_launch - launched);
         chunk_start[slot] = launched;
         chunk_n[slot] = timing ? n : 0;
         for (int t = 0; t < n; t++) {
@@ -423,14 +433,19 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
             if (timing)
                 DROT_HIP_CHECK(hipEventRecord(g_tpool.ev[(size_t)(slot * chunk + t) * 2], stream));
             if (P.kind == kQR) {
+                qa.ctl.k = k;
                 DROT_HIP_CHECK(launch_qr_pass(P.qt, (k % 2) == 0, P.grid_pass, qa, stream));
             } else {
+                ga.ctl.k = k;
                 DROT_HIP_CHECK(launch_gl_pass(P.gt, P.grid_pass, P.lds, ga, stream));
             }
             if (timing)
                 DROT_HIP_CHECK(
                     hipEventRecord(g_tpool.ev[(size_t)(slot * chunk + t) * 2 + 1], stream));
-            DROT_HIP_CHECK(launch_finalize(grid_fin, fa, stream));
+            if (k < in.max_iters) {
+                fa.with_obj = (P.kind == kGL) || ((k % 2) == 0);
+                DROT_HIP_CHECK(launch_finalize(grid_fin, fa, stream));
+            }
         }
         launched += n;
         DROT_HIP_CHECK(hipMemcpyAsync((void *)&flags[slot], &d.st->done, sizeof(int),
@@ -466,7 +481,7 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
     if (res) {
         res->n_iter = hs->done ? hs->n_iter : (P.kind == kQR ? in.max_iters + 1 : in.max_iters);
         res->converged = hs->converged;
-        res->iterations_run = hs->k;
+        res->iterations_run = hs->k_run;
         res->objective = (float)hs->obj;
         res->objective_f64 = hs->obj;
         res->residual = hs->res;
@@ -475,7 +490,7 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
         res->pass_ms[0] = res->pass_ms[1] = 0.0;
         res->pass_count[0] = res->pass_count[1] = 0;
         if (timing) {
-            for (int k = 0; k < hs->k && k < (int)pass_ms.size(); k++) {
+            for (int k = 0; k < hs->k_run && k < (int)pass_ms.size(); k++) {
                 const int par = (P.kind == kQR && (k % 2) == 1) ? 1 : 0;
                 res->pass_ms[par] += pass_ms[(size_t)k];
                 res->pass_count[par] += 1;
