@@ -22,6 +22,7 @@ namespace drot {
 constexpr int kWave = 64;          // CDNA wavefront width
 constexpr int kBlock = 256;        // threads per block for every solver kernel
 constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kXcds = 8;           // accelerator complex dies (separate L2s) on MI355X
 
 // Solver state kept in device memory for the whole solve.  The host never
 // reads it inside the iteration loop except through asynchronous snapshots.

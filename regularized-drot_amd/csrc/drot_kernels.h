@@ -1,6 +1,7 @@
 // Host-visible launch interface of the solver kernels (drot_kernels.hip).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
@@ -76,10 +77,13 @@ struct GlTile {
 };
 
 hipError_t launch_init(const InitArgs &a, hipStream_t s);
+// e0/e1 (optional): events recorded at the pass kernel's start and end.
 hipError_t launch_qr_pass(const QrTile &t, bool even, int grid, const QrPassArgs &a,
-                          hipStream_t s);
+                          hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+int qr_pass_grid(int n_rt, int n_ct);
 hipError_t launch_gl_pass(const GlTile &t, int grid, size_t lds_bytes,
-                          const GlPassArgs &a, hipStream_t s);
+                          const GlPassArgs &a, hipStream_t s, hipEvent_t e0 = nullptr,
+                          hipEvent_t e1 = nullptr);
 hipError_t launch_finalize(int grid, const FinalizeArgs &a, hipStream_t s);
 hipError_t launch_fix_x(float *x, const float *c, float step, size_t total,
                         const SolverState *st, hipStream_t s);
@@ -91,7 +95,7 @@ hipError_t launch_export_phi(const float *ahat, const float *bhat, const float *
 bool qr_tile_supported(const QrTile &t);
 
 // rows (or columns) reduced per finalize block
-constexpr int kFinRows = 64;
+constexpr int kFinRows = 32;
 bool gl_tile_supported(const GlTile &t);
 
 }  // namespace drot

@@ -379,11 +379,15 @@ bool gl_tile_supported(const GlTile &t) {
 }
 
 hipError_t launch_gl_pass(const GlTile &t, int grid, size_t lds, const GlPassArgs &a,
-                          hipStream_t s) {
-#define DROT_GL_CASE(V, R)                                                              \
-    if (t.vec == V && t.rch == R) {                                                     \
-        hipLaunchKernelGGL((gl_pass_kernel<V, R>), dim3(grid), dim3(kBlock), lds, s, a); \
-        return hipGetLastError();                                                       \
+                          hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+#define DROT_GL_CASE(V, R)                                                                \
+    if (t.vec == V && t.rch == R) {                                                       \
+        if (e0 && e1)                                                                     \
+            hipExtLaunchKernelGGL((gl_pass_kernel<V, R>), dim3(grid), dim3(kBlock), lds, s, \
+                                  e0, e1, 0, a);                                          \
+        else                                                                              \
+            hipLaunchKernelGGL((gl_pass_kernel<V, R>), dim3(grid), dim3(kBlock), lds, s, a); \
+        return hipGetLastError();                                                         \
     }
     DROT_GL_CASE(4, 1) DROT_GL_CASE(4, 2) DROT_GL_CASE(4, 4) DROT_GL_CASE(4, 8)
     DROT_GL_CASE(1, 1) DROT_GL_CASE(1, 2) DROT_GL_CASE(1, 4) DROT_GL_CASE(1, 8)

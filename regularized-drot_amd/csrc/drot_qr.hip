@@ -20,17 +20,22 @@ namespace drot {
 namespace {
 
 // 16-byte / 4-byte global accesses.  Store policy ST: 0 = default (L2
-// write-back), 1 = nontemporal, 2 = device-scope write-through (sc1),
-// 3 = system-scope write-through (sc0 sc1).  Write-through keeps the streamed
-// X out of the L2 dirty set, so the agent-scope release of the tile tails only
-// has the small partial buffers to write back.
+// write-back global store), 2 = device-scope write-through (sc1) buffer store
+// through a per-tile buffer resource whose range is the tile's columns, so a
+// store outside the tile is dropped by the hardware range check.
+// Write-through keeps the streamed X out of the L2 dirty set, which shortens
+// the end-of-kernel cache write-back.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-template <int VEC, int ST>
+constexpr int kBufDword3 = 0x00020000;  // raw buffer, 32-bit data format
+constexpr int kPolicySc1 = 16;          // cache-policy operand: sc1
+
+template <int VEC>
 struct Io;
 
-template <int ST>
-struct Io<4, ST> {
+template <>
+struct Io<4> {
     static __device__ __forceinline__ void load(const float *p, float (&v)[4]) {
         const f32x4 t = *reinterpret_cast<const f32x4 *>(p);
         v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
@@ -38,37 +43,32 @@ struct Io<4, ST> {
     static __device__ __forceinline__ void store(float *p, const float (&v)[4]) {
         f32x4 t;
         t.x = v[0]; t.y = v[1]; t.z = v[2]; t.w = v[3];
-        if (ST == 1) {
-            __builtin_nontemporal_store(t, reinterpret_cast<f32x4 *>(p));
-        } else if (ST == 2) {
-            asm volatile("global_store_dwordx4 %0, %1, off sc1" ::"v"(p), "v"(t) : "memory");
-        } else if (ST == 3) {
-            asm volatile("global_store_dwordx4 %0, %1, off sc0 sc1" ::"v"(p), "v"(t) : "memory");
-        } else {
-            *reinterpret_cast<f32x4 *>(p) = t;
-        }
+        *reinterpret_cast<f32x4 *>(p) = t;
+    }
+    static __device__ __forceinline__ void store_wt(__amdgpu_buffer_rsrc_t r, int byte_off,
+                                                    const float (&v)[4]) {
+        u32x4 d;
+        d.x = __float_as_uint(v[0]);
+        d.y = __float_as_uint(v[1]);
+        d.z = __float_as_uint(v[2]);
+        d.w = __float_as_uint(v[3]);
+        __builtin_amdgcn_raw_buffer_store_b128(d, r, byte_off, 0, kPolicySc1);
     }
 };
 
-template <int ST>
-struct Io<1, ST> {
+template <>
+struct Io<1> {
     static __device__ __forceinline__ void load(const float *p, float (&v)[1]) { v[0] = *p; }
-    static __device__ __forceinline__ void store(float *p, const float (&v)[1]) {
-        if (ST == 1) {
-            __builtin_nontemporal_store(v[0], p);
-        } else if (ST == 2) {
-            asm volatile("global_store_dword %0, %1, off sc1" ::"v"(p), "v"(v[0]) : "memory");
-        } else if (ST == 3) {
-            asm volatile("global_store_dword %0, %1, off sc0 sc1" ::"v"(p), "v"(v[0]) : "memory");
-        } else {
-            *p = v[0];
-        }
+    static __device__ __forceinline__ void store(float *p, const float (&v)[1]) { *p = v[0]; }
+    static __device__ __forceinline__ void store_wt(__amdgpu_buffer_rsrc_t r, int byte_off,
+                                                    const float (&v)[1]) {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v[0]), r, byte_off, 0, kPolicySc1);
     }
 };
 
 }  // namespace
 
-// ST: store policy of X (see Io).
+// ST: store policy of X (see Io); ST == 2 needs tc * n_rows * 4 < 2^31.
 //
 // Software pipelined: the first batch of X/C loads (and the row potentials)
 // is issued before the prologue's reductions, and every batch's loads are in
@@ -86,8 +86,16 @@ __global__ __launch_bounds__(kBlock) void qr_pass_kernel(QrPassArgs A) {
     const int n_rt = A.n_rt;
     const int nstep = A.nstep;
     const float fM = (float)M, fN = (float)N;
-    const int rt = blockIdx.x % n_rt;
-    const int ct = blockIdx.x / n_rt;
+    // XCD-aware tile order: blocks are dispatched round-robin over the 8 XCDs
+    // (block b on XCD b % 8), so give each XCD a contiguous run of tiles in
+    // row-tile-major order.  Vertically adjacent tiles then share an L2 and the
+    // cache lines straddling their boundary are fetched once.
+    const int n_tiles = n_rt * A.n_ct;
+    const int per_xcd = (n_tiles + kXcds - 1) / kXcds;
+    const int tile = (int)(blockIdx.x % kXcds) * per_xcd + (int)(blockIdx.x / kXcds);
+    if (tile >= n_tiles) return;
+    const int rt = tile % n_rt;
+    const int ct = tile / n_rt;
     const int tc = nstep * kWavesPerBlock * CPW;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = wave_uniform(threadIdx.x / kWave);
@@ -110,8 +118,8 @@ __global__ __launch_bounds__(kBlock) void qr_pass_kernel(QrPassArgs A) {
             h2[u] = col_ok ? hat2[j] : 0.0f;
             if (col_ok && row_ok) {
                 const size_t off = (size_t)j * (size_t)N + (size_t)row;
-                Io<VEC, ST>::load(A.x + off, xv[u]);
-                if (EVEN) Io<VEC, ST>::load(A.c + off, cv[u]);
+                Io<VEC>::load(A.x + off, xv[u]);
+                if (EVEN) Io<VEC>::load(A.c + off, cv[u]);
                 else {
 #pragma unroll
                     for (int e = 0; e < VEC; e++) cv[u][e] = 0.0f;
@@ -127,6 +135,12 @@ __global__ __launch_bounds__(kBlock) void qr_pass_kernel(QrPassArgs A) {
 #pragma unroll
     for (int e = 0; e < VEC; e++)
         h1[e] = row_ok ? (explicit_phi ? A.phi1[row + e] : A.ahat[row + e]) : 0.0f;
+
+    // write-through stores address the tile's column range [tile_col0, +tc)
+    const int tile_col0 = ct * tc;
+    const int tile_cols = min(tc, M - tile_col0);
+    __amdgpu_buffer_rsrc_t xrsrc = __builtin_amdgcn_make_buffer_rsrc(
+        A.x + (size_t)tile_col0 * (size_t)N, 0, tile_cols * N * 4, kBufDword3);
 
     const IterStart is = iteration_prologue(A.ctl, s_red);
     if (is.finish) return;
@@ -168,8 +182,12 @@ __global__ __launch_bounds__(kBlock) void qr_pass_kernel(QrPassArgs A) {
                 }
             }
             if (ok) {
-                const size_t off = (size_t)j * (size_t)N + (size_t)row;
-                Io<VEC, ST>::store(A.x + off, xo);
+                if (ST == 2) {
+                    Io<VEC>::store_wt(xrsrc, ((j - tile_col0) * N + row) * 4, xo);
+                } else {
+                    const size_t off = (size_t)j * (size_t)N + (size_t)row;
+                    Io<VEC>::store(A.x + off, xo);
+                }
             }
             csum = seg_sum<LPC>(csum);
             if (rl == 0 && col_ok) A.pcol[(size_t)rt * M + j] = csum;
@@ -196,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void qr_pass_kernel(QrPassArgs A) {
     }
     if (EVEN) {
         const double o = block_sum(oacc, s_red);
-        if (threadIdx.x == 0) A.pobj[blockIdx.x] = o;
+        if (threadIdx.x == 0) A.pobj[tile] = o;
     }
 }
 
@@ -208,35 +226,42 @@ bool qr_tile_supported(const QrTile &t) {
 }
 
 template <int VEC, int LPC, int ST, int U>
-static void launch_pass_t(bool even, int grid, const QrPassArgs &a, hipStream_t s) {
-    if (even)
-        hipLaunchKernelGGL((qr_pass_kernel<VEC, LPC, true, ST, U>), dim3(grid), dim3(kBlock), 0, s, a);
+static void launch_pass_t(bool even, int grid, const QrPassArgs &a, hipStream_t s,
+                          hipEvent_t e0, hipEvent_t e1) {
+    auto k = even ? qr_pass_kernel<VEC, LPC, true, ST, U> : qr_pass_kernel<VEC, LPC, false, ST, U>;
+    if (e0 && e1)  // timing: events bracket the kernel's own execution
+        hipExtLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, a);
     else
-        hipLaunchKernelGGL((qr_pass_kernel<VEC, LPC, false, ST, U>), dim3(grid), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, a);
 }
 
 template <int VEC, int LPC>
 static void launch_pass_v(const QrTile &t, bool even, int grid, const QrPassArgs &a,
-                          hipStream_t s) {
+                          hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if (t.nt == 2) {
-        if (t.u == 2) launch_pass_t<VEC, LPC, 2, 2>(even, grid, a, s);
-        else launch_pass_t<VEC, LPC, 2, 4>(even, grid, a, s);
+        if (t.u == 2) launch_pass_t<VEC, LPC, 2, 2>(even, grid, a, s, e0, e1);
+        else launch_pass_t<VEC, LPC, 2, 4>(even, grid, a, s, e0, e1);
     } else {
-        if (t.u == 2) launch_pass_t<VEC, LPC, 0, 2>(even, grid, a, s);
-        else launch_pass_t<VEC, LPC, 0, 4>(even, grid, a, s);
+        if (t.u == 2) launch_pass_t<VEC, LPC, 0, 2>(even, grid, a, s, e0, e1);
+        else launch_pass_t<VEC, LPC, 0, 4>(even, grid, a, s, e0, e1);
     }
 }
 
+int qr_pass_grid(int n_rt, int n_ct) {
+    const int per = (n_rt * n_ct + kXcds - 1) / kXcds;
+    return per * kXcds;
+}
+
 hipError_t launch_qr_pass(const QrTile &t, bool even, int grid, const QrPassArgs &a,
-                          hipStream_t s) {
+                          hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if (t.vec == 4) {
-        if (t.lpc == 16) launch_pass_v<4, 16>(t, even, grid, a, s);
-        else if (t.lpc == 32) launch_pass_v<4, 32>(t, even, grid, a, s);
-        else launch_pass_v<4, 64>(t, even, grid, a, s);
+        if (t.lpc == 16) launch_pass_v<4, 16>(t, even, grid, a, s, e0, e1);
+        else if (t.lpc == 32) launch_pass_v<4, 32>(t, even, grid, a, s, e0, e1);
+        else launch_pass_v<4, 64>(t, even, grid, a, s, e0, e1);
     } else {
-        if (t.lpc == 16) launch_pass_v<1, 16>(t, even, grid, a, s);
-        else if (t.lpc == 32) launch_pass_v<1, 32>(t, even, grid, a, s);
-        else launch_pass_v<1, 64>(t, even, grid, a, s);
+        if (t.lpc == 16) launch_pass_v<1, 16>(t, even, grid, a, s, e0, e1);
+        else if (t.lpc == 32) launch_pass_v<1, 32>(t, even, grid, a, s, e0, e1);
+        else launch_pass_v<1, 64>(t, even, grid, a, s, e0, e1);
     }
     return hipGetLastError();
 }

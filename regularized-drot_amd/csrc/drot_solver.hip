@@ -128,8 +128,10 @@ Plan plan_qr(int n_rows, int n_cols, bool vec4_ok) {
     P.nstep = nstep;
     P.tc = nstep * tc_min;
     P.n_ct = ceil_div(n_cols, P.tc);
-    P.grid_pass = P.n_rt * P.n_ct;
-    P.n_obj = P.grid_pass;
+    // the write-through store path addresses one column tile with 32-bit offsets
+    if ((long long)P.tc * n_rows * 4 >= (1LL << 31)) P.qt.nt = 0;
+    P.grid_pass = qr_pass_grid(P.n_rt, P.n_ct);
+    P.n_obj = P.n_rt * P.n_ct;
     P.rb = ceil_div(n_rows, kFinRows);
     P.cb = ceil_div(n_cols, kFinRows);
     layout(P);
@@ -430,18 +432,18 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
         chunk_n[slot] = timing ? n : 0;
         for (int t = 0; t < n; t++) {
             const int k = launched + t;
-            if (timing)
-                DROT_HIP_CHECK(hipEventRecord(g_tpool.ev[(size_t)(slot * chunk + t) * 2], stream));
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (timing) {
+                e0 = g_tpool.ev[(size_t)(slot * chunk + t) * 2];
+                e1 = g_tpool.ev[(size_t)(slot * chunk + t) * 2 + 1];
+            }
             if (P.kind == kQR) {
                 qa.ctl.k = k;
-                DROT_HIP_CHECK(launch_qr_pass(P.qt, (k % 2) == 0, P.grid_pass, qa, stream));
+                DROT_HIP_CHECK(launch_qr_pass(P.qt, (k % 2) == 0, P.grid_pass, qa, stream, e0, e1));
             } else {
                 ga.ctl.k = k;
-                DROT_HIP_CHECK(launch_gl_pass(P.gt, P.grid_pass, P.lds, ga, stream));
+                DROT_HIP_CHECK(launch_gl_pass(P.gt, P.grid_pass, P.lds, ga, stream, e0, e1));
             }
-            if (timing)
-                DROT_HIP_CHECK(
-                    hipEventRecord(g_tpool.ev[(size_t)(slot * chunk + t) * 2 + 1], stream));
             if (k < in.max_iters) {
                 fa.with_obj = (P.kind == kGL) || ((k % 2) == 0);
                 DROT_HIP_CHECK(launch_finalize(grid_fin, fa, stream));
