@@ -139,6 +139,42 @@ drot_status drot_export_duals_f32(int kind, int n_rows, int n_cols, int n_groups
                                   const void *workspace, size_t workspace_bytes,
                                   float *phi1, float *phi2, float *alpha, void *stream);
 
+/* ---------------------------------------------------------------- batches */
+
+/* One problem of a batch.  All problems' C (and X) live in one packed device
+ * buffer: problem i is column-major n_rows x n_cols with leading dimension ld
+ * (ld >= n_rows, ld % 4 == 0; rows n_rows..ld-1 are scratch) starting at float
+ * offset mat_offset (a multiple of 4).  p and q are packed the same way at
+ * p_offset / q_offset.  params are per problem (drot_qr_params_from_*). */
+typedef struct drot_batch_problem {
+    int n_rows, n_cols;
+    int ld;
+    long long mat_offset;
+    long long p_offset, q_offset;
+    drot_qr_params params;
+} drot_batch_problem;
+
+/* Fill shapes and a dense packed layout (ld = n_rows rounded up to 4) for a
+ * batch; params are left to the caller.  Outputs the packed lengths. */
+drot_status drot_batch_layout(int n_problems, const int *n_rows, const int *n_cols,
+                              drot_batch_problem *out, long long *mat_elems,
+                              long long *p_elems, long long *q_elems);
+
+drot_status drot_qr_batch_workspace_size(int n_problems, const drot_batch_problem *problems,
+                                         size_t *bytes);
+
+/* Solve many QR-DROT problems of uneven shape together: one pass launch and
+ * one finalize launch per iteration over a work-list of the still-active
+ * problems (compacted as problems converge), each problem with its own
+ * stopping test.  Replaces the reference's sequential loop over problems
+ * (core/multiexp_qr.cu:35-55 calling quadratic_regularizer_drot_wrapper per
+ * problem); each problem's result is identical to drot_qr_solve_f32 on it.
+ * results: host array of n_problems (may be NULL). */
+drot_status drot_qr_batch_solve_f32(int n_problems, const drot_batch_problem *problems,
+                                    const float *c, const float *p, const float *q, float *x,
+                                    void *workspace, size_t workspace_bytes, void *stream,
+                                    drot_result *results);
+
 /* ---------------------------------------------------------------- host API */
 
 /* Host-pointer wrapper with the argument order of

@@ -132,15 +132,16 @@ struct IterStart {
 //   gamma = (2 (S - 1) - alpha) / (n_rows + n_cols)   (alpha of iteration k-2)
 //   alpha <- (alpha + 1) - S                          (core/kernel_qr.hpp:421,463-467)
 // and applies the stopping test of iteration k-1 (core/drot_qr.hpp:217-229 or
-// core/drot_glr.hpp:344-354).  Block 0 records the result.  Because the test
+// core/drot_glr.hpp:344-354).  The problem's leader block records the result.  Because the test
 // runs before any block writes X, a converged iterate is never overwritten.
-__device__ inline IterStart iteration_prologue(const IterCtl &C, double *s_red) {
+__device__ inline IterStart iteration_prologue(const IterCtl &C, double *s_red,
+                                              bool leader_block) {
     SolverState *st = C.st;
     IterStart r;
     r.explicit_phi = true;
     r.gamma = 0.0f;
     const int k = C.k;
-    const bool leader = (blockIdx.x == 0) && (threadIdx.x == 0);
+    const bool leader = leader_block && (threadIdx.x == 0);
     bool conv = false;
     bool finish = k >= C.max_iters;
     if (k > 0) {

@@ -94,6 +94,59 @@ hipError_t launch_export_phi(const float *ahat, const float *bhat, const float *
 
 bool qr_tile_supported(const QrTile &t);
 
+// ------------------------------------------------------------------ batches
+
+// One problem of a batch as the device sees it.  Matrices are column-major
+// with leading dimension ld (ld % 4 == 0, mat_off % 4 == 0, padding rows are
+// scratch); row vectors (p, phi1, a, ahat) live at row_off, column vectors at
+// col_off of the packed arrays.
+struct BatchProblem {
+    long long mat_off;
+    long long row_off, col_off;      // packed p / q (and dual vector) offsets
+    long long prow_off, pcol_off;    // offsets into the partial buffers
+    int n_rows, n_cols, ld;
+    int n_rt, n_ct, nstep;
+    int tile_off;                    // objective partials: pobj[tile_off + rt + ct * n_rt]
+    int fin_off, n_fin, rb;          // finalize blocks: rb row blocks, then column blocks
+    int max_iters;
+    float step, kappa, eps;
+    float v_phi1, v_phi2, v_a, v_b, v_alpha;  // warm-start constants
+    int warm;
+};
+
+struct BatchTile {
+    int prob, rt, ct, pad;
+};
+
+struct BatchFin {
+    int prob, blk;
+};
+
+struct QrBatchArgs {
+    float *x;
+    const float *c, *p, *q;
+    float *phi1, *phi2, *a, *b, *ahat, *bhat;  // packed dual vectors
+    double *prow, *pcol, *pobj, *pfin;
+    SolverState *st;                           // one per problem
+    const BatchProblem *probs;
+    const BatchTile *tiles;
+    int n_tiles;
+    const BatchFin *fins;
+    int n_fins;
+    int n_probs;
+    int k;                                     // iteration of this launch
+};
+
+// Batched launches use LPC = 16 lanes per column, 16-byte accesses (tile rows 64).
+constexpr int kBatchLpc = 16;
+constexpr int kBatchTileRows = kBatchLpc * 4;
+
+int batch_pass_grid(int n_tiles);
+hipError_t launch_batch_init(const QrBatchArgs &a, hipStream_t s);
+hipError_t launch_batch_pass(bool even, const QrBatchArgs &a, hipStream_t s);
+hipError_t launch_batch_finalize(const QrBatchArgs &a, hipStream_t s);
+hipError_t launch_batch_fix_x(const QrBatchArgs &a, hipStream_t s);
+
 // rows (or columns) reduced per finalize block
 constexpr int kFinRows = 32;
 bool gl_tile_supported(const GlTile &t);

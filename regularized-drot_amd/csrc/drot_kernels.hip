@@ -10,6 +10,7 @@
 //                       and the stopping test -- all on device, no host round trip.
 // The pass is HBM-bound elementwise work (about 1 flop per byte), so it uses
 // plain VALU, 128-bit coalesced accesses and no matrix cores.
+#include "drot_finalize.h"
 #include "drot_kernels.h"
 
 #include <math.h>
@@ -113,7 +114,7 @@ __global__ __launch_bounds__(kBlock) void gl_pass_kernel(GlPassArgs A) {
     extern __shared__ double s_dyn[];
     __shared__ double s_red[3 * kWavesPerBlock];
     if (A.ctl.st->done) return;
-    const IterStart is = iteration_prologue(A.ctl, s_red);
+    const IterStart is = iteration_prologue(A.ctl, s_red, blockIdx.x == 0);
     if (is.finish) return;
     const int phi_mode = is.explicit_phi ? 1 : 0;
     const float gamma = is.gamma;
@@ -235,84 +236,31 @@ __global__ __launch_bounds__(kBlock) void gl_pass_kernel(GlPassArgs A) {
 
 // ----------------------------------------------------------------- finalize
 
-// Dual update of iteration k, shared by QR and GL: the row and column halves
-// of quadratic_regularizer_update_aux (core/kernel_qr.hpp:405-461, identical
-// in core/kernel_glr.hpp:461-520) plus the residual reductions of
-// core/drot_qr.hpp:145-156:
-//   rowsum = sum_ct prow ; rs = rowsum - p ; a <- a - rs ; ahat = a - rs
-//   colsum = sum_rt pcol ; cs = colsum - q ; b <- b - cs ; bhat = b - cs
-// Blocks [0, rb) own kFinRows rows each, [rb, rb+cb) kFinRows columns; four
-// threads share a row so all partial loads of a row are in flight at once.
-// Each block leaves {sum rs^2 or cs^2, sum X', objective slice} in pfin; the
-// scalar part (gamma, alpha, residual, stopping test) is formed from those by
-// the prologue of the next pass, so this kernel has no grid-wide step and no
-// agent-scope fence.
 __global__ __launch_bounds__(kBlock) void finalize_kernel(FinalizeArgs A) {
-    constexpr int kParts = kBlock / kFinRows;
-    __shared__ double s_part[kParts][kFinRows];
+    __shared__ double s_part[kBlock / kFinRows][kFinRows];
     __shared__ double s_red[3 * kWavesPerBlock];
     if (A.st->done) return;
-    const int N = A.n_rows, M = A.n_cols;
-    const int b = blockIdx.x;
-    const int part = threadIdx.x / kFinRows;
-    const int li = threadIdx.x % kFinRows;
-    const bool is_row = b < A.rb;
-    const int idx = (is_row ? b : b - A.rb) * kFinRows + li;
-    const int len = is_row ? N : M;
-    const int cnt = is_row ? A.n_ct : A.n_rt;
-    const bool valid = idx < len;
-    const double *src = (is_row ? A.prow : A.pcol) + (valid ? idx : 0);
-    double sum = 0.0;
-    if (valid) {
-        int t = part;
-        for (; t + (8 - 1) * kParts < cnt; t += 8 * kParts) {
-            double v[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = src[(size_t)(t + u * kParts) * len];
-#pragma unroll
-            for (int u = 0; u < 8; u++) sum += v[u];
-        }
-        for (; t < cnt; t += kParts) sum += src[(size_t)t * len];
-    }
-    s_part[part][li] = sum;
-    __syncthreads();
-    double res = 0.0, S = 0.0;
-    if (part == 0 && valid) {
-        double tot = 0.0;
-#pragma unroll
-        for (int q = 0; q < kParts; q++) tot += s_part[q][li];
-        if (is_row) {
-            const double d = tot - (double)A.p[idx];
-            const float rs = (float)d;
-            const float an = A.a[idx] - rs;
-            A.a[idx] = an;
-            A.ahat[idx] = an - rs;
-            res = d * d;
-        } else {
-            const double d = tot - (double)A.q[idx];
-            const float cs = (float)d;
-            const float bn = A.b[idx] - cs;
-            A.b[idx] = bn;
-            A.bhat[idx] = bn - cs;
-            res = d * d;
-            S = tot;
-        }
-    }
-    // this block's slice of the pass's objective partials
-    double obj = 0.0;
-    if (A.with_obj) {
-        const int n_blk = A.rb + A.cb;
-        const int per = (A.n_obj + n_blk - 1) / n_blk;
-        const int o0 = b * per;
-        const int o1 = min(o0 + per, A.n_obj);
-        for (int t = o0 + threadIdx.x; t < o1; t += kBlock) obj += A.pobj[t];
-    }
-    block_sum3(res, S, obj, s_red);
-    if (threadIdx.x == 0) {
-        A.pfin[3 * b] = res;
-        A.pfin[3 * b + 1] = S;
-        A.pfin[3 * b + 2] = obj;
-    }
+    FinJob F;
+    F.prow = A.prow;
+    F.pcol = A.pcol;
+    F.pobj = A.pobj;
+    F.p = A.p;
+    F.q = A.q;
+    F.a = A.a;
+    F.b = A.b;
+    F.ahat = A.ahat;
+    F.bhat = A.bhat;
+    F.pfin = A.pfin + 3 * blockIdx.x;
+    F.n_rows = A.n_rows;
+    F.n_cols = A.n_cols;
+    F.n_rt = A.n_rt;
+    F.n_ct = A.n_ct;
+    F.n_obj = A.n_obj;
+    F.rb = A.rb;
+    F.n_fin = A.rb + A.cb;
+    F.blk = blockIdx.x;
+    F.with_obj = A.with_obj;
+    finalize_block(F, s_part, s_red);
 }
 
 // ----------------------------------------------------------------- fix_x

@@ -1,0 +1,220 @@
+// QR-DROT pass over one tile (gfx950), shared by the single-problem and the
+// batched pass kernels so both run exactly the same per-element arithmetic.
+//
+// Restates quadratic_regularizer_update_x_{even,odd} (core/kernel_qr.hpp:171-376):
+//   X' = max(X + phi1 + phi2 - sC, 0)/kappa,  stored X = X' - sC (even k) | X'
+//   (odd k, whose kernel never reads C: the -sC is already folded into X),
+//   phi = (hat + gamma)/n formed on the fly from the finalize kernel's output.
+// A tile is TR = LPC*VEC rows by tc columns of one problem; LPC lanes cover
+// the tile rows of one column with 16-byte accesses, so a wave handles 64/LPC
+// columns per step.  Column partials: butterfly over those lanes.  Row
+// partials: fp64 registers across the tile's columns, combined through LDS in
+// a fixed order.  Objective partials on even k.
+//
+// Software pipelined: the first batch of X/C loads (and the row potentials)
+// is issued before the prologue's reductions, and every batch's loads are in
+// flight while the previous batch is computed and stored.
+#pragma once
+
+#include "drot_kernels.h"
+
+namespace drot {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kBufDword3 = 0x00020000;  // raw buffer, 32-bit data format
+constexpr int kPolicySc1 = 16;          // cache-policy operand: sc1
+
+// 16-byte / 4-byte global accesses.  Store policy ST: 0 = default (L2
+// write-back global store), 2 = device-scope write-through (sc1) buffer store
+// through a per-tile buffer resource whose range is the tile's columns, so a
+// store outside the tile is dropped by the hardware range check.
+// Write-through keeps the streamed X out of the L2 dirty set, which shortens
+// the end-of-kernel cache write-back.
+template <int VEC>
+struct Io;
+
+template <>
+struct Io<4> {
+    static __device__ __forceinline__ void load(const float *p, float (&v)[4]) {
+        const f32x4 t = *reinterpret_cast<const f32x4 *>(p);
+        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    }
+    static __device__ __forceinline__ void store(float *p, const float (&v)[4]) {
+        f32x4 t;
+        t.x = v[0]; t.y = v[1]; t.z = v[2]; t.w = v[3];
+        *reinterpret_cast<f32x4 *>(p) = t;
+    }
+    static __device__ __forceinline__ void store_wt(__amdgpu_buffer_rsrc_t r, int byte_off,
+                                                    const float (&v)[4]) {
+        u32x4 d;
+        d.x = __float_as_uint(v[0]);
+        d.y = __float_as_uint(v[1]);
+        d.z = __float_as_uint(v[2]);
+        d.w = __float_as_uint(v[3]);
+        __builtin_amdgcn_raw_buffer_store_b128(d, r, byte_off, 0, kPolicySc1);
+    }
+};
+
+template <>
+struct Io<1> {
+    static __device__ __forceinline__ void load(const float *p, float (&v)[1]) { v[0] = *p; }
+    static __device__ __forceinline__ void store(float *p, const float (&v)[1]) { *p = v[0]; }
+    static __device__ __forceinline__ void store_wt(__amdgpu_buffer_rsrc_t r, int byte_off,
+                                                    const float (&v)[1]) {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v[0]), r, byte_off, 0, kPolicySc1);
+    }
+};
+
+// Everything one tile needs.  x/c point at the problem's matrix (leading
+// dimension ld >= n_rows, ld % VEC == 0), the vectors at the problem's slices.
+struct QrTileJob {
+    float *x;
+    const float *c;
+    const float *phi1, *phi2, *ahat, *bhat;
+    double *prow;   // problem's [n_ct][n_rows] row partials
+    double *pcol;   // problem's [n_rt][n_cols] column partials
+    double *pobj;   // this tile's objective partial
+    int n_rows, n_cols, ld;
+    int rt, ct, tc, nstep;
+    float step, kappa;
+};
+
+template <int LPC>
+struct QrTileShared {
+    static constexpr int kCpw = kWave / LPC;
+};
+
+template <int VEC, int LPC, bool EVEN, int ST, int U>
+__device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, bool leader_block,
+                                        double (*s_row)[LPC * VEC], double *s_red) {
+    constexpr int CPW = kWave / LPC;  // columns per wave step
+    constexpr int TR = LPC * VEC;     // tile rows
+    constexpr int kColStride = kWavesPerBlock * CPW;
+    const int N = J.n_rows, M = J.n_cols, ld = J.ld;
+    const int nstep = J.nstep;
+    const float fM = (float)M, fN = (float)N;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = wave_uniform(threadIdx.x / kWave);
+    const int cs = lane / LPC;
+    const int rl = lane % LPC;
+    const int row = J.rt * TR + rl * VEC;
+    const bool row_ok = row < N;
+    bool ev_ok[VEC];  // element rows inside the problem (padding rows excluded)
+#pragma unroll
+    for (int e = 0; e < VEC; e++) ev_ok[e] = row + e < N;
+    const int tile_col0 = J.ct * J.tc;
+    const int col0 = tile_col0 + w * CPW + cs;  // column of step t: col0 + t * kColStride
+    const bool explicit_phi = (ctl.k == 0);
+    const float *hat2 = explicit_phi ? J.phi2 : J.bhat;
+
+    float xa[U][VEC], ca[U][VEC], h2a[U];
+    auto load_batch = [&](int t0, float (&xv)[U][VEC], float (&cv)[U][VEC], float (&h2)[U]) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u;
+            const int j = col0 + t * kColStride;
+            const bool col_ok = (t < nstep) && (j < M);
+            h2[u] = col_ok ? hat2[j] : 0.0f;
+            if (col_ok && row_ok) {
+                const size_t off = (size_t)j * (size_t)ld + (size_t)row;
+                Io<VEC>::load(J.x + off, xv[u]);
+                if (EVEN) Io<VEC>::load(J.c + off, cv[u]);
+                else {
+#pragma unroll
+                    for (int e = 0; e < VEC; e++) cv[u][e] = 0.0f;
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < VEC; e++) { xv[u][e] = 0.0f; cv[u][e] = 0.0f; }
+            }
+        }
+    };
+    load_batch(0, xa, ca, h2a);
+    float h1[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; e++)
+        h1[e] = ev_ok[e] ? (explicit_phi ? J.phi1[row + e] : J.ahat[row + e]) : 0.0f;
+    // write-through stores address the tile's column range [tile_col0, +tc)
+    const int tile_cols = min(J.tc, M - tile_col0);
+    __amdgpu_buffer_rsrc_t xrsrc = __builtin_amdgcn_make_buffer_rsrc(
+        J.x + (size_t)tile_col0 * (size_t)ld, 0, tile_cols * ld * 4, kBufDword3);
+
+    const IterStart is = iteration_prologue(ctl, s_red, leader_block);
+    if (is.finish) return;
+    const float gamma = is.gamma;
+    const float s = J.step, kappa = J.kappa;
+
+    float f1[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; e++) f1[e] = explicit_phi ? h1[e] : (h1[e] + gamma) / fM;
+    double racc[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; e++) racc[e] = 0.0;
+    double oacc = 0.0;
+
+    for (int t0 = 0; t0 < nstep; t0 += U) {
+        float xb[U][VEC], cb[U][VEC], h2b[U];
+        if (t0 + U < nstep) load_batch(t0 + U, xb, cb, h2b);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u;
+            if (t >= nstep) break;  // wave-uniform
+            const int j = col0 + t * kColStride;
+            const bool col_ok = j < M;
+            const bool ok = col_ok && row_ok;
+            const float f2 = explicit_phi ? h2a[u] : (h2a[u] + gamma) / fN;
+            float xo[VEC];
+            double csum = 0.0;
+#pragma unroll
+            for (int e = 0; e < VEC; e++) {
+                const float tt = (xa[u][e] + f1[e]) + f2;
+                float y = EVEN ? __builtin_fmaf(-s, ca[u][e], tt) : tt;
+                y = fmaxf(y, 0.0f);
+                const float xp = y / kappa;
+                xo[e] = EVEN ? __builtin_fmaf(-s, ca[u][e], xp) : xp;
+                if (ok && ev_ok[e]) {
+                    racc[e] += (double)xp;
+                    csum += (double)xp;
+                    if (EVEN) oacc += (double)xp * (double)ca[u][e];
+                }
+            }
+            if (ok) {
+                if (ST == 2) {
+                    Io<VEC>::store_wt(xrsrc, ((j - tile_col0) * ld + row) * 4, xo);
+                } else {
+                    const size_t off = (size_t)j * (size_t)ld + (size_t)row;
+                    Io<VEC>::store(J.x + off, xo);
+                }
+            }
+            csum = seg_sum<LPC>(csum);
+            if (rl == 0 && col_ok) J.pcol[(size_t)J.rt * M + j] = csum;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            h2a[u] = h2b[u];
+#pragma unroll
+            for (int e = 0; e < VEC; e++) { xa[u][e] = xb[u][e]; ca[u][e] = cb[u][e]; }
+        }
+    }
+
+    // strided-axis partials: fixed-order combine of the 4*CPW column slots
+    const int slot = w * CPW + cs;
+#pragma unroll
+    for (int e = 0; e < VEC; e++) s_row[slot][rl * VEC + e] = racc[e];
+    __syncthreads();
+    for (int r = threadIdx.x; r < TR; r += kBlock) {
+        double sum = 0.0;
+#pragma unroll
+        for (int q = 0; q < kWavesPerBlock * CPW; q++) sum += s_row[q][r];
+        const int i = J.rt * TR + r;
+        if (i < N) J.prow[(size_t)J.ct * N + i] = sum;
+    }
+    if (EVEN) {
+        const double o = block_sum(oacc, s_red);
+        if (threadIdx.x == 0) *J.pobj = o;
+    }
+}
+
+}  // namespace drot

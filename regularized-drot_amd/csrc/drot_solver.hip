@@ -828,3 +828,328 @@ float *drot_glr_wrapper_f32(const float *c, const float *p, const float *q, int 
 }
 
 }  // extern "C"
+
+// ====================================================================== batches
+
+namespace drot {
+namespace {
+
+struct BatchPlan {
+    std::vector<BatchProblem> probs;
+    std::vector<BatchTile> tiles;  // all tiles, problem by problem
+    std::vector<BatchFin> fins;    // all finalize blocks, problem by problem
+    std::vector<int> tile_begin, fin_begin;  // per problem, into tiles / fins
+    long long row_len = 0, col_len = 0, prow_len = 0, pcol_len = 0;
+    int max_iters = 0;
+    size_t off_state = 0, off_vec = 0, off_prow = 0, off_pcol = 0, off_pobj = 0, off_pfin = 0;
+    size_t off_probs = 0, off_tiles = 0, off_fins = 0, total = 0;
+};
+
+drot_status plan_batch(int n, const drot_batch_problem *in, BatchPlan *out) {
+    if (n <= 0 || !in) return fail(DROT_ERR_INVALID_ARGUMENT, "empty batch");
+    BatchPlan B;
+    constexpr int cpw = kWave / kBatchLpc;
+    constexpr int tc_min = kWavesPerBlock * cpw;
+    int tile_off = 0, fin_off = 0;
+    long long prow_off = 0, pcol_off = 0;
+    for (int i = 0; i < n; i++) {
+        const drot_batch_problem &d = in[i];
+        if (d.n_rows <= 0 || d.n_cols <= 0)
+            return fail(DROT_ERR_INVALID_ARGUMENT, "problem " + std::to_string(i) + ": bad shape");
+        if (d.ld < d.n_rows || d.ld % 4 != 0 || d.mat_offset % 4 != 0 || d.mat_offset < 0 ||
+            d.p_offset < 0 || d.q_offset < 0)
+            return fail(DROT_ERR_INVALID_ARGUMENT,
+                        "problem " + std::to_string(i) +
+                            ": ld must be >= n_rows and a multiple of 4, offsets multiples of 4");
+        if (!(d.params.scale > 0.0f) || d.params.max_iters < 0)
+            return fail(DROT_ERR_INVALID_ARGUMENT, "problem " + std::to_string(i) + ": bad params");
+        BatchProblem P{};
+        P.mat_off = d.mat_offset;
+        P.row_off = d.p_offset;
+        P.col_off = d.q_offset;
+        P.n_rows = d.n_rows;
+        P.n_cols = d.n_cols;
+        P.ld = d.ld;
+        P.n_rt = ceil_div(d.n_rows, kBatchTileRows);
+        P.nstep = std::max(1, 4096 / (kBatchTileRows * tc_min));
+        const int tc = P.nstep * tc_min;
+        if ((long long)tc * d.ld * 4 >= (1LL << 31))
+            return fail(DROT_ERR_UNSUPPORTED, "problem " + std::to_string(i) + ": ld too large");
+        P.n_ct = ceil_div(d.n_cols, tc);
+        P.prow_off = prow_off;
+        P.pcol_off = pcol_off;
+        prow_off += (long long)P.n_ct * d.n_rows;
+        pcol_off += (long long)P.n_rt * d.n_cols;
+        P.tile_off = tile_off;
+        P.rb = ceil_div(d.n_rows, kFinRows);
+        P.n_fin = P.rb + ceil_div(d.n_cols, kFinRows);
+        P.fin_off = fin_off;
+        P.max_iters = d.params.max_iters;
+        P.step = d.params.step_size;
+        P.kappa = d.params.scale;
+        P.eps = d.params.eps;
+        P.warm = d.params.warm_start ? 1 : 0;
+        InitArgs ia{};
+        warm_constants(d.n_rows, d.n_cols, d.params.step_size, &ia);
+        P.v_phi1 = ia.v_phi1;
+        P.v_phi2 = ia.v_phi2;
+        P.v_a = ia.v_a;
+        P.v_b = ia.v_b;
+        P.v_alpha = ia.v_alpha;
+        B.tile_begin.push_back((int)B.tiles.size());
+        for (int ct = 0; ct < P.n_ct; ct++)
+            for (int rt = 0; rt < P.n_rt; rt++) B.tiles.push_back(BatchTile{i, rt, ct, 0});
+        B.fin_begin.push_back((int)B.fins.size());
+        for (int b = 0; b < P.n_fin; b++) B.fins.push_back(BatchFin{i, b});
+        tile_off += P.n_rt * P.n_ct;
+        fin_off += P.n_fin;
+        B.row_len = std::max(B.row_len, d.p_offset + d.n_rows);
+        B.col_len = std::max(B.col_len, d.q_offset + d.n_cols);
+        B.max_iters = std::max(B.max_iters, P.max_iters);
+        B.probs.push_back(P);
+    }
+    B.tile_begin.push_back((int)B.tiles.size());
+    B.fin_begin.push_back((int)B.fins.size());
+    B.prow_len = prow_off;
+    B.pcol_len = pcol_off;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = align_up(off + bytes, 256);
+        return o;
+    };
+    B.off_state = take(sizeof(SolverState) * (size_t)n);
+    B.off_vec = take(sizeof(float) * 3 * (size_t)(B.row_len + B.col_len));
+    B.off_prow = take(8 * (size_t)B.prow_len);
+    B.off_pcol = take(8 * (size_t)B.pcol_len);
+    B.off_pobj = take(8 * (size_t)tile_off);
+    B.off_pfin = take(24 * (size_t)fin_off);
+    B.off_probs = take(sizeof(BatchProblem) * (size_t)n);
+    B.off_tiles = take(sizeof(BatchTile) * B.tiles.size());
+    B.off_fins = take(sizeof(BatchFin) * B.fins.size());
+    B.total = off;
+    *out = std::move(B);
+    return DROT_OK;
+}
+
+// Pinned staging for batch work-lists and state snapshots (per thread).
+struct BatchStaging {
+    void *buf[2] = {nullptr, nullptr};
+    size_t cap = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int device = -1;
+};
+thread_local BatchStaging g_bstage;
+
+drot_status batch_staging(size_t bytes, BatchStaging **out) {
+    int dev = 0;
+    DROT_HIP_CHECK(hipGetDevice(&dev));
+    BatchStaging &S = g_bstage;
+    if (S.device != dev || S.cap < bytes) {
+        // buffers of a previous device or size are kept alive (process lifetime)
+        S = BatchStaging();
+        const size_t cap = std::max<size_t>(bytes, 1 << 20);
+        for (int i = 0; i < 2; i++) {
+            DROT_HIP_CHECK(hipHostMalloc(&S.buf[i], cap, hipHostMallocDefault));
+            DROT_HIP_CHECK(hipEventCreateWithFlags(&S.ev[i], hipEventDisableTiming));
+        }
+        S.cap = cap;
+        S.device = dev;
+    }
+    *out = &S;
+    return DROT_OK;
+}
+
+drot_status run_batch(const BatchPlan &B, const float *c, const float *p, const float *q,
+                      float *x, void *ws, hipStream_t stream, drot_result *results) {
+    const int n = (int)B.probs.size();
+    char *base = static_cast<char *>(ws);
+    QrBatchArgs A{};
+    A.x = x;
+    A.c = c;
+    A.p = p;
+    A.q = q;
+    float *vec = reinterpret_cast<float *>(base + B.off_vec);
+    A.phi1 = vec;
+    A.a = vec + B.row_len;
+    A.ahat = vec + 2 * B.row_len;
+    A.phi2 = vec + 3 * B.row_len;
+    A.b = A.phi2 + B.col_len;
+    A.bhat = A.phi2 + 2 * B.col_len;
+    A.prow = reinterpret_cast<double *>(base + B.off_prow);
+    A.pcol = reinterpret_cast<double *>(base + B.off_pcol);
+    A.pobj = reinterpret_cast<double *>(base + B.off_pobj);
+    A.pfin = reinterpret_cast<double *>(base + B.off_pfin);
+    A.st = reinterpret_cast<SolverState *>(base + B.off_state);
+    A.probs = reinterpret_cast<const BatchProblem *>(base + B.off_probs);
+    BatchTile *d_tiles = reinterpret_cast<BatchTile *>(base + B.off_tiles);
+    BatchFin *d_fins = reinterpret_cast<BatchFin *>(base + B.off_fins);
+    A.tiles = d_tiles;
+    A.fins = d_fins;
+    A.n_tiles = (int)B.tiles.size();
+    A.n_fins = (int)B.fins.size();
+    A.n_probs = n;
+
+    const size_t list_bytes =
+        sizeof(BatchTile) * B.tiles.size() + sizeof(BatchFin) * B.fins.size();
+    const size_t snap_bytes = sizeof(SolverState) * (size_t)n;
+    BatchStaging *S = nullptr;
+    drot_status ss =
+        batch_staging(std::max(list_bytes, sizeof(BatchProblem) * (size_t)n) + snap_bytes + 4096, &S);
+    if (ss != DROT_OK) return ss;
+
+    // problem table and full work-lists (synchronous: once per solve)
+    DROT_HIP_CHECK(hipMemcpyAsync(base + B.off_probs, B.probs.data(), sizeof(BatchProblem) * n,
+                                  hipMemcpyHostToDevice, stream));
+    DROT_HIP_CHECK(hipMemcpyAsync(d_tiles, B.tiles.data(), sizeof(BatchTile) * B.tiles.size(),
+                                  hipMemcpyHostToDevice, stream));
+    DROT_HIP_CHECK(hipMemcpyAsync(d_fins, B.fins.data(), sizeof(BatchFin) * B.fins.size(),
+                                  hipMemcpyHostToDevice, stream));
+    DROT_HIP_CHECK(launch_batch_init(A, stream));
+    DROT_HIP_CHECK(hipStreamSynchronize(stream));
+
+    // iteration chunks; after each chunk an asynchronous snapshot of the
+    // problem states, read one chunk later, drives work-list compaction
+    long long active_bytes = 0;
+    for (const auto &P : B.probs) active_bytes += (long long)P.n_rows * P.n_cols * 10;
+    std::vector<char> active(n, 1);
+    int n_active = n;
+    const int n_launch = B.max_iters + 1;
+    int launched = 0, slot = 0;
+    bool pending[2] = {false, false};
+    SolverState *snap[2];
+    for (int h = 0; h < 2; h++)
+        snap[h] = reinterpret_cast<SolverState *>(static_cast<char *>(S->buf[h]) + list_bytes +
+                                                  4096 - (list_bytes % 256));
+    while (launched < n_launch && n_active > 0) {
+        const double us = std::max(6.0, (double)active_bytes / 5.0e6);
+        int chunk = std::max(2, std::min(64, (int)(1000.0 / us))) & ~1;
+        const int cnt = std::min(chunk, n_launch - launched);
+        for (int t = 0; t < cnt; t++) {
+            A.k = launched + t;
+            DROT_HIP_CHECK(launch_batch_pass((A.k % 2) == 0, A, stream));
+            if (A.k < B.max_iters) DROT_HIP_CHECK(launch_batch_finalize(A, stream));
+        }
+        launched += cnt;
+        DROT_HIP_CHECK(hipMemcpyAsync(snap[slot], A.st, snap_bytes, hipMemcpyDeviceToHost, stream));
+        DROT_HIP_CHECK(hipEventRecord(S->ev[slot], stream));
+        pending[slot] = true;
+        const int other = slot ^ 1;
+        if (pending[other]) {
+            DROT_HIP_CHECK(hipEventSynchronize(S->ev[other]));
+            pending[other] = false;
+            bool changed = false;
+            for (int i = 0; i < n; i++) {
+                if (active[i] && snap[other][i].done) {
+                    active[i] = 0;
+                    n_active--;
+                    changed = true;
+                }
+            }
+            if (changed && n_active > 0) {
+                // compact the work-lists to the active problems
+                BatchTile *lt = static_cast<BatchTile *>(S->buf[other]);
+                int nt = 0;
+                active_bytes = 0;
+                for (int i = 0; i < n; i++) {
+                    if (!active[i]) continue;
+                    for (int t = B.tile_begin[i]; t < B.tile_begin[i + 1]; t++) lt[nt++] = B.tiles[t];
+                    active_bytes += (long long)B.probs[i].n_rows * B.probs[i].n_cols * 10;
+                }
+                BatchFin *lf = reinterpret_cast<BatchFin *>(lt + nt);
+                int nf = 0;
+                for (int i = 0; i < n; i++) {
+                    if (!active[i]) continue;
+                    for (int f = B.fin_begin[i]; f < B.fin_begin[i + 1]; f++) lf[nf++] = B.fins[f];
+                }
+                DROT_HIP_CHECK(hipMemcpyAsync(d_tiles, lt, sizeof(BatchTile) * nt,
+                                              hipMemcpyHostToDevice, stream));
+                DROT_HIP_CHECK(hipMemcpyAsync(d_fins, lf, sizeof(BatchFin) * nf,
+                                              hipMemcpyHostToDevice, stream));
+                A.n_tiles = nt;
+                A.n_fins = nf;
+            }
+        }
+        // Staging half `other` is written again only after the snapshot of the
+        // next chunk in that half has been synchronised, and that snapshot is
+        // ordered after this upload on the stream, so no extra wait is needed.
+        slot = other;
+    }
+    DROT_HIP_CHECK(launch_batch_fix_x(A, stream));
+    DROT_HIP_CHECK(hipMemcpyAsync(snap[0], A.st, snap_bytes, hipMemcpyDeviceToHost, stream));
+    DROT_HIP_CHECK(hipStreamSynchronize(stream));
+    if (results) {
+        for (int i = 0; i < n; i++) {
+            const SolverState &h = snap[0][i];
+            drot_result &r = results[i];
+            memset(&r, 0, sizeof(r));
+            r.n_iter = h.done ? h.n_iter : B.probs[i].max_iters + 1;
+            r.converged = h.converged;
+            r.iterations_run = h.k_run;
+            r.objective = (float)h.obj;
+            r.objective_f64 = h.obj;
+            r.residual = h.res;
+        }
+    }
+    return DROT_OK;
+}
+
+}  // namespace
+}  // namespace drot
+
+extern "C" {
+
+drot_status drot_batch_layout(int n_problems, const int *n_rows, const int *n_cols,
+                              drot_batch_problem *out, long long *mat_elems,
+                              long long *p_elems, long long *q_elems) {
+    if (n_problems <= 0 || !n_rows || !n_cols || !out)
+        return fail(DROT_ERR_INVALID_ARGUMENT, "bad arguments to drot_batch_layout");
+    long long m = 0, pe = 0, qe = 0;
+    for (int i = 0; i < n_problems; i++) {
+        if (n_rows[i] <= 0 || n_cols[i] <= 0)
+            return fail(DROT_ERR_INVALID_ARGUMENT, "bad shape in drot_batch_layout");
+        drot_batch_problem &d = out[i];
+        d.n_rows = n_rows[i];
+        d.n_cols = n_cols[i];
+        d.ld = (n_rows[i] + 3) / 4 * 4;
+        d.mat_offset = m;
+        d.p_offset = pe;
+        d.q_offset = qe;
+        m += (long long)d.ld * n_cols[i];
+        pe += n_rows[i];
+        qe += n_cols[i];
+    }
+    if (mat_elems) *mat_elems = m;
+    if (p_elems) *p_elems = pe;
+    if (q_elems) *q_elems = qe;
+    return DROT_OK;
+}
+
+drot_status drot_qr_batch_workspace_size(int n_problems, const drot_batch_problem *problems,
+                                         size_t *bytes) {
+    if (!bytes) return fail(DROT_ERR_INVALID_ARGUMENT, "bytes is NULL");
+    BatchPlan B;
+    const drot_status st = plan_batch(n_problems, problems, &B);
+    if (st != DROT_OK) return st;
+    *bytes = B.total;
+    return DROT_OK;
+}
+
+drot_status drot_qr_batch_solve_f32(int n_problems, const drot_batch_problem *problems,
+                                    const float *c, const float *p, const float *q, float *x,
+                                    void *workspace, size_t workspace_bytes, void *stream,
+                                    drot_result *results) {
+    if (!c || !p || !q || !x || !workspace)
+        return fail(DROT_ERR_INVALID_ARGUMENT, "null pointer argument");
+    if (((reinterpret_cast<uintptr_t>(c) | reinterpret_cast<uintptr_t>(x)) & 15u) != 0)
+        return fail(DROT_ERR_INVALID_ARGUMENT, "c and x must be 16-byte aligned");
+    BatchPlan B;
+    const drot_status st = plan_batch(n_problems, problems, &B);
+    if (st != DROT_OK) return st;
+    if (workspace_bytes < B.total)
+        return fail(DROT_ERR_WORKSPACE_TOO_SMALL,
+                    "workspace needs " + std::to_string(B.total) + " bytes");
+    return run_batch(B, c, p, q, x, workspace, static_cast<hipStream_t>(stream), results);
+}
+
+}  // extern "C"

@@ -26,6 +26,7 @@ EXPORTS = (
     "drot_qr_solve_f32", "drot_glr_solve_f32", "drot_export_duals_f32",
     "drot_qr_wrapper_f32", "drot_glr_wrapper_f32", "drot_free",
     "drot_status_string", "drot_last_error", "drot_abi_version", "drot_set_timing",
+    "drot_batch_layout", "drot_qr_batch_workspace_size", "drot_qr_batch_solve_f32",
 )
 
 DROT_OK = 0
@@ -41,6 +42,12 @@ class GlrParams(ctypes.Structure):
     _fields_ = [("step_size", ctypes.c_float), ("lambda_", ctypes.c_float),
                 ("n_groups", ctypes.c_int), ("max_iters", ctypes.c_int),
                 ("eps", ctypes.c_float), ("warm_start", ctypes.c_int)]
+
+
+class BatchProblem(ctypes.Structure):
+    _fields_ = [("n_rows", ctypes.c_int), ("n_cols", ctypes.c_int), ("ld", ctypes.c_int),
+                ("mat_offset", ctypes.c_longlong), ("p_offset", ctypes.c_longlong),
+                ("q_offset", ctypes.c_longlong), ("params", QrParams)]
 
 
 class Result(ctypes.Structure):
@@ -107,10 +114,17 @@ def load(path: str = LIB_PATH):
     lib.drot_abi_version.restype = c_int
     lib.drot_set_timing.argtypes = [c_int]
     lib.drot_set_timing.restype = None
+    ll = ctypes.c_longlong
+    lib.drot_batch_layout.argtypes = [c_int, P(c_int), P(c_int), P(BatchProblem), P(ll), P(ll),
+                                      P(ll)]
+    lib.drot_qr_batch_workspace_size.argtypes = [c_int, P(BatchProblem), P(c_size)]
+    lib.drot_qr_batch_solve_f32.argtypes = [c_int, P(BatchProblem), vp, vp, vp, vp, vp, c_size,
+                                            vp, P(Result)]
     for name in ("drot_qr_params_from_rho", "drot_qr_params_from_step",
                  "drot_glr_params_from_rho", "drot_glr_params_from_step",
                  "drot_qr_workspace_size", "drot_glr_workspace_size",
-                 "drot_qr_solve_f32", "drot_glr_solve_f32", "drot_export_duals_f32"):
+                 "drot_qr_solve_f32", "drot_glr_solve_f32", "drot_export_duals_f32",
+                 "drot_batch_layout", "drot_qr_batch_workspace_size", "drot_qr_batch_solve_f32"):
         getattr(lib, name).restype = c_int
     _lib = lib
     return lib
@@ -192,3 +206,31 @@ def export_duals(kind, n_rows, n_cols, n_groups, ws, ws_bytes, phi1, phi2, strea
                                        phi2, ctypes.byref(alpha), stream),
           "drot_export_duals_f32")
     return alpha.value
+
+
+def batch_layout(n_rows, n_cols):
+    """Packed layout of a batch: (problems array, mat_elems, p_elems, q_elems)."""
+    n = len(n_rows)
+    probs = (BatchProblem * n)()
+    nr = (ctypes.c_int * n)(*[int(v) for v in n_rows])
+    nc = (ctypes.c_int * n)(*[int(v) for v in n_cols])
+    m, pe, qe = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+    check(load().drot_batch_layout(n, nr, nc, probs, ctypes.byref(m), ctypes.byref(pe),
+                                   ctypes.byref(qe)), "drot_batch_layout")
+    return probs, m.value, pe.value, qe.value
+
+
+def batch_workspace_size(probs) -> int:
+    b = ctypes.c_size_t()
+    check(load().drot_qr_batch_workspace_size(len(probs), probs, ctypes.byref(b)),
+          "drot_qr_batch_workspace_size")
+    return b.value
+
+
+def batch_solve(probs, c, p, q, x, ws, ws_bytes, stream):
+    """Device addresses for c/p/q/x/ws; returns a list of Result."""
+    n = len(probs)
+    res = (Result * n)()
+    check(load().drot_qr_batch_solve_f32(n, probs, c, p, q, x, ws, ws_bytes, stream, res),
+          "drot_qr_batch_solve_f32")
+    return list(res)
