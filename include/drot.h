@@ -175,6 +175,62 @@ drot_status drot_qr_batch_solve_f32(int n_problems, const drot_batch_problem *pr
                                     void *workspace, size_t workspace_bytes, void *stream,
                                     drot_result *results);
 
+/* ---------------------------------------------------------------- backward */
+
+/* Options of the implicit-differentiation backward (all fields 0 = default). */
+typedef struct drot_backward_params {
+    float support_tol;      /* (i,j) is in the support when x_ij > 0 and
+                               x_ij > support_tol * step_size * |c_ij|; default
+                               2^-22, which rejects the rounding residue that
+                               adding step*C back to the stored X - step*C
+                               leaves where the plan is zero */
+    int max_cg_iters;       /* default 4 * (n_rows + n_cols) + 100 */
+    double cg_tol;          /* relative CG residual target; default 1e-10 */
+    long long nnz_capacity; /* index capacity per problem; default n_rows*n_cols
+                               up to 2^22, else max(2^22, 64 (n_rows + n_cols)) */
+} drot_backward_params;
+
+typedef struct drot_backward_info {
+    long long nnz;          /* support size */
+    int cg_iters;
+    int converged;          /* CG reached cg_tol */
+    double cg_residual;     /* final ||r|| / ||g|| */
+} drot_backward_info;
+
+/* Gradient of a loss L(X*, F) of a QR-DROT plan X* and of its regularized
+ * objective F = <C,X*> + (reg/2)||X*||^2, reg = (scale - 1) / step_size, with
+ * respect to C:
+ *   dL/dC = S o (w 1^T + 1 w'^T - G) / reg + g_F X*,  G = dL/dX*,  g_F = dL/dF,
+ * where S is the support of X* and [w; w'] solves the bipartite-Laplacian system
+ * of the KKT conditions on S (Jacobi-preconditioned CG in fp64).  The reference
+ * has no backward: its binding returns a tensor without gradient
+ * (pytorch/src/qr_drot_cuda_kernel.cu:99), so this entry point has no
+ * counterpart there.  c, x (the plan returned by drot_qr_solve_f32), grad_x and
+ * grad_c are device matrices in the solver layout; grad_obj is a device pointer
+ * to one float (NULL = 0).  params: the ones the plan was solved with.
+ * bparams and info may be NULL.  Synchronous on `stream`. */
+drot_status drot_qr_backward_workspace_size(int n_rows, int n_cols, const drot_qr_params *params,
+                                            const drot_backward_params *bparams, size_t *bytes);
+drot_status drot_qr_backward_f32(const float *c, const float *x, const float *grad_x,
+                                 const float *grad_obj, int n_rows, int n_cols,
+                                 const drot_qr_params *params, const drot_backward_params *bparams,
+                                 float *grad_c, void *workspace, size_t workspace_bytes,
+                                 void *stream, drot_backward_info *info);
+
+/* Batched backward over packed problems (layout of drot_qr_batch_solve_f32;
+ * ld needs no rounding here).  grad_obj: device array of n_problems floats or
+ * NULL.  Padding rows of grad_c are set to 0.  One count and one fill launch
+ * over all problems' column tiles, then one CG block per problem. */
+drot_status drot_qr_batch_backward_workspace_size(int n_problems,
+                                                  const drot_batch_problem *problems,
+                                                  const drot_backward_params *bparams,
+                                                  size_t *bytes);
+drot_status drot_qr_batch_backward_f32(int n_problems, const drot_batch_problem *problems,
+                                       const float *c, const float *x, const float *grad_x,
+                                       const float *grad_obj, const drot_backward_params *bparams,
+                                       float *grad_c, void *workspace, size_t workspace_bytes,
+                                       void *stream, drot_backward_info *infos);
+
 /* ---------------------------------------------------------------- host API */
 
 /* Host-pointer wrapper with the argument order of

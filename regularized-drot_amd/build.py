@@ -33,7 +33,8 @@ HIP_FLAGS = [
     "-Wall", "-I", INCLUDE,
 ]
 
-SOURCES = ["drot_kernels.hip", "drot_qr.hip", "drot_batch.hip", "drot_solver.hip"]
+SOURCES = ["drot_kernels.hip", "drot_qr.hip", "drot_batch.hip", "drot_backward.hip",
+           "drot_solver.hip"]
 
 
 def _run(cmd, quiet_filter=()):

@@ -151,4 +151,56 @@ hipError_t launch_batch_fix_x(const QrBatchArgs &a, hipStream_t s);
 constexpr int kFinRows = 32;
 bool gl_tile_supported(const GlTile &t);
 
+// ------------------------------------------------------- implicit backward
+//
+// dL/dC of a converged QR-DROT plan by implicit differentiation of its KKT
+// conditions on the support S (see drot_backward.hip).  One problem or a
+// batch; every matrix is column-major n_rows x n_cols with leading dimension
+// ld at float offset mat_off of the packed buffers.
+
+struct BwdProblem {
+    long long mat_off;
+    long long iptr_off;   // colptr[n_cols + 1], rowptr[n_rows + 1], rowcur[n_rows]
+    long long idx_off;    // rowidx[cap] (CSC order), colidx[cap] (CSR order)
+    long long vec_off;    // 5 fp64 vectors of n_rows + n_cols: w, r, z, p, q
+    long long cap;        // index capacity (< 2^31)
+    int n_rows, n_cols, ld;
+    int tile_begin, n_tiles;
+    int max_cg;
+    float thr;            // support: x > 0 and x > thr * |c|  (thr = tol * step)
+    float pad;
+    double reg;           // quadratic weight of the problem: (kappa - 1) / step
+    double cg_tol;        // relative residual target of the CG solve
+};
+
+struct BwdTile {
+    int prob, c0, c1, pad;
+};
+
+struct alignas(64) BwdState {
+    long long nnz;        // support size (may exceed cap: overflow)
+    int cg_iters;
+    int converged;
+    double res;           // final ||r|| / ||g||
+    double gnorm;         // ||g||, g = (row sums, column sums) of G on S
+};
+
+struct BwdArgs {
+    const float *c, *x, *g;
+    const float *gobj;    // [n_probs] device scalars (grad of the objective) or null
+    float *grad_c;
+    const BwdProblem *probs;
+    const BwdTile *tiles;
+    BwdState *st;
+    int *tile_cnt;
+    int *iptr;
+    int *idx;
+    double *vec;
+    int n_tiles, n_probs;
+};
+
+hipError_t launch_bwd_count(const BwdArgs &a, hipStream_t s);
+hipError_t launch_bwd_fill(const BwdArgs &a, hipStream_t s);
+hipError_t launch_bwd_cg(const BwdArgs &a, hipStream_t s);
+
 }  // namespace drot

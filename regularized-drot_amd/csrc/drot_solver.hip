@@ -19,6 +19,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -1094,6 +1095,165 @@ drot_status run_batch(const BatchPlan &B, const float *c, const float *p, const 
     return DROT_OK;
 }
 
+
+// ------------------------------------------------------------ backward plan
+
+struct BwdPlan {
+    std::vector<BwdProblem> probs;
+    std::vector<BwdTile> tiles;
+    size_t off_probs = 0, off_tiles = 0, off_state = 0, off_tcnt = 0, off_iptr = 0;
+    size_t off_idx = 0, off_vec = 0, total = 0;
+};
+
+// Default index capacity: the whole matrix up to 2^22 elements, beyond that
+// max(2^22, 64 (n_rows + n_cols)) entries (supports of QR plans hold a few
+// entries per row; a denser support reports DROT_ERR_WORKSPACE_TOO_SMALL and
+// drot_backward_params.nnz_capacity raises the limit).
+long long bwd_capacity(const drot_batch_problem &d, const drot_backward_params *bp) {
+    const long long mn = (long long)d.n_rows * d.n_cols;
+    long long cap;
+    if (bp && bp->nnz_capacity > 0) cap = std::min<long long>(bp->nnz_capacity, mn);
+    else cap = std::min<long long>(mn, std::max<long long>(1LL << 22, 64LL * (d.n_rows + d.n_cols)));
+    return std::min<long long>(cap, 0x7fffffffLL - 1);
+}
+
+drot_status plan_bwd(int n, const drot_batch_problem *in, const drot_backward_params *bp,
+                     BwdPlan *out) {
+    if (n <= 0 || !in) return fail(DROT_ERR_INVALID_ARGUMENT, "empty problem list");
+    const double tol = (bp && bp->support_tol > 0.0f) ? (double)bp->support_tol : 0x1p-22;
+    const double cg_tol = (bp && bp->cg_tol > 0.0) ? bp->cg_tol : 1e-10;
+    BwdPlan B;
+    long long iptr = 0, idx = 0, vec = 0;
+    for (int i = 0; i < n; i++) {
+        const drot_batch_problem &d = in[i];
+        const std::string who = "problem " + std::to_string(i);
+        if (d.n_rows <= 0 || d.n_cols <= 0) return fail(DROT_ERR_INVALID_ARGUMENT, who + ": bad shape");
+        if (d.ld < d.n_rows || d.mat_offset < 0)
+            return fail(DROT_ERR_INVALID_ARGUMENT, who + ": ld must be >= n_rows, offsets >= 0");
+        if ((long long)d.n_rows + d.n_cols >= (1LL << 30))
+            return fail(DROT_ERR_UNSUPPORTED, who + ": too many rows and columns");
+        const double step = d.params.step_size, kappa = d.params.scale;
+        const double reg = (kappa - 1.0) / step;
+        if (!(step > 0.0) || !(reg > 0.0) || !std::isfinite(reg))
+            return fail(DROT_ERR_INVALID_ARGUMENT,
+                        who + ": the backward needs step_size > 0 and scale > 1 (r_weight > 0)");
+        BwdProblem P{};
+        P.mat_off = d.mat_offset;
+        P.n_rows = d.n_rows;
+        P.n_cols = d.n_cols;
+        P.ld = d.ld;
+        P.cap = bwd_capacity(d, bp);
+        P.iptr_off = iptr;
+        iptr += (long long)d.n_cols + 1 + 2LL * d.n_rows + 1;
+        P.idx_off = idx;
+        idx += 2 * P.cap;
+        P.vec_off = vec;
+        vec += 5LL * (d.n_rows + d.n_cols);
+        const long long L = (long long)d.n_rows + d.n_cols;
+        P.max_cg = (bp && bp->max_cg_iters > 0) ? bp->max_cg_iters
+                                                 : (int)std::min<long long>(4 * L + 100, 1 << 30);
+        P.thr = (float)(tol * step);
+        P.reg = reg;
+        P.cg_tol = cg_tol;
+        // column tiles of about 16K elements
+        const int cpt = std::max(1, std::min(d.n_cols, 16384 / std::max(1, d.ld)));
+        P.tile_begin = (int)B.tiles.size();
+        for (int c0 = 0; c0 < d.n_cols; c0 += cpt)
+            B.tiles.push_back(BwdTile{i, c0, std::min(d.n_cols, c0 + cpt), 0});
+        P.n_tiles = (int)B.tiles.size() - P.tile_begin;
+        if (B.tiles.size() >= (size_t)(1u << 31))
+            return fail(DROT_ERR_UNSUPPORTED, "too many column tiles");
+        B.probs.push_back(P);
+    }
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = align_up(off + bytes, 256);
+        return o;
+    };
+    B.off_probs = take(sizeof(BwdProblem) * B.probs.size());
+    B.off_tiles = take(sizeof(BwdTile) * B.tiles.size());
+    B.off_state = take(sizeof(BwdState) * B.probs.size());
+    B.off_tcnt = take(sizeof(int) * B.tiles.size());
+    B.off_iptr = take(sizeof(int) * (size_t)iptr);
+    B.off_idx = take(sizeof(int) * (size_t)idx);
+    B.off_vec = take(sizeof(double) * (size_t)vec);
+    B.total = off;
+    *out = std::move(B);
+    return DROT_OK;
+}
+
+drot_status run_bwd(const BwdPlan &B, const float *c, const float *x, const float *g,
+                    const float *gobj, float *grad_c, void *ws, size_t ws_bytes,
+                    hipStream_t stream, drot_backward_info *infos) {
+    if (!c || !x || !g || !grad_c || !ws) return fail(DROT_ERR_INVALID_ARGUMENT, "null pointer argument");
+    if (ws_bytes < B.total)
+        return fail(DROT_ERR_WORKSPACE_TOO_SMALL, "workspace needs " + std::to_string(B.total) + " bytes");
+    char *base = static_cast<char *>(ws);
+    const int n = (int)B.probs.size();
+    BwdArgs A{};
+    A.c = c;
+    A.x = x;
+    A.g = g;
+    A.gobj = gobj;
+    A.grad_c = grad_c;
+    A.probs = reinterpret_cast<const BwdProblem *>(base + B.off_probs);
+    A.tiles = reinterpret_cast<const BwdTile *>(base + B.off_tiles);
+    A.st = reinterpret_cast<BwdState *>(base + B.off_state);
+    A.tile_cnt = reinterpret_cast<int *>(base + B.off_tcnt);
+    A.iptr = reinterpret_cast<int *>(base + B.off_iptr);
+    A.idx = reinterpret_cast<int *>(base + B.off_idx);
+    A.vec = reinterpret_cast<double *>(base + B.off_vec);
+    A.n_tiles = (int)B.tiles.size();
+    A.n_probs = n;
+    BatchStaging *S = nullptr;
+    const size_t tab = sizeof(BwdProblem) * B.probs.size();
+    const size_t til = sizeof(BwdTile) * B.tiles.size();
+    const size_t stb = sizeof(BwdState) * B.probs.size();
+    drot_status ss = batch_staging(align_up(tab, 256) + align_up(til, 256) + stb, &S);
+    if (ss != DROT_OK) return ss;
+    char *h = static_cast<char *>(S->buf[0]);
+    memcpy(h, B.probs.data(), tab);
+    memcpy(h + align_up(tab, 256), B.tiles.data(), til);
+    BwdState *hs = reinterpret_cast<BwdState *>(h + align_up(tab, 256) + align_up(til, 256));
+    DROT_HIP_CHECK(hipMemcpyAsync(base + B.off_probs, h, tab, hipMemcpyHostToDevice, stream));
+    DROT_HIP_CHECK(hipMemcpyAsync(base + B.off_tiles, h + align_up(tab, 256), til,
+                                  hipMemcpyHostToDevice, stream));
+    DROT_HIP_CHECK(launch_bwd_count(A, stream));
+    DROT_HIP_CHECK(launch_bwd_fill(A, stream));
+    DROT_HIP_CHECK(launch_bwd_cg(A, stream));
+    DROT_HIP_CHECK(hipMemcpyAsync(hs, A.st, stb, hipMemcpyDeviceToHost, stream));
+    DROT_HIP_CHECK(hipStreamSynchronize(stream));
+    int overflow = -1;
+    for (int i = 0; i < n; i++) {
+        if (hs[i].nnz > B.probs[i].cap && overflow < 0) overflow = i;
+        if (infos) {
+            drot_backward_info &r = infos[i];
+            memset(&r, 0, sizeof(r));
+            r.nnz = hs[i].nnz;
+            r.cg_iters = hs[i].cg_iters;
+            r.converged = hs[i].converged;
+            r.cg_residual = hs[i].res;
+        }
+    }
+    if (overflow >= 0)
+        return fail(DROT_ERR_WORKSPACE_TOO_SMALL,
+                    "problem " + std::to_string(overflow) + ": support of " +
+                        std::to_string(hs[overflow].nnz) + " entries exceeds the index capacity " +
+                        std::to_string(B.probs[overflow].cap) +
+                        " (raise drot_backward_params.nnz_capacity)");
+    return DROT_OK;
+}
+
+drot_batch_problem single_problem(int n_rows, int n_cols, const drot_qr_params *prm) {
+    drot_batch_problem d{};
+    d.n_rows = n_rows;
+    d.n_cols = n_cols;
+    d.ld = n_rows;
+    d.params = *prm;
+    return d;
+}
+
 }  // namespace
 }  // namespace drot
 
@@ -1150,6 +1310,55 @@ drot_status drot_qr_batch_solve_f32(int n_problems, const drot_batch_problem *pr
         return fail(DROT_ERR_WORKSPACE_TOO_SMALL,
                     "workspace needs " + std::to_string(B.total) + " bytes");
     return run_batch(B, c, p, q, x, workspace, static_cast<hipStream_t>(stream), results);
+}
+
+drot_status drot_qr_backward_workspace_size(int n_rows, int n_cols, const drot_qr_params *params,
+                                            const drot_backward_params *bparams, size_t *bytes) {
+    if (!bytes || !params) return fail(DROT_ERR_INVALID_ARGUMENT, "null pointer argument");
+    const drot_batch_problem d = single_problem(n_rows, n_cols, params);
+    BwdPlan B;
+    const drot_status st = plan_bwd(1, &d, bparams, &B);
+    if (st != DROT_OK) return st;
+    *bytes = B.total;
+    return DROT_OK;
+}
+
+drot_status drot_qr_backward_f32(const float *c, const float *x, const float *grad_x,
+                                 const float *grad_obj, int n_rows, int n_cols,
+                                 const drot_qr_params *params, const drot_backward_params *bparams,
+                                 float *grad_c, void *workspace, size_t workspace_bytes,
+                                 void *stream, drot_backward_info *info) {
+    if (!params) return fail(DROT_ERR_INVALID_ARGUMENT, "params is NULL");
+    const drot_batch_problem d = single_problem(n_rows, n_cols, params);
+    BwdPlan B;
+    const drot_status st = plan_bwd(1, &d, bparams, &B);
+    if (st != DROT_OK) return st;
+    return run_bwd(B, c, x, grad_x, grad_obj, grad_c, workspace, workspace_bytes,
+                   static_cast<hipStream_t>(stream), info);
+}
+
+drot_status drot_qr_batch_backward_workspace_size(int n_problems,
+                                                  const drot_batch_problem *problems,
+                                                  const drot_backward_params *bparams,
+                                                  size_t *bytes) {
+    if (!bytes) return fail(DROT_ERR_INVALID_ARGUMENT, "bytes is NULL");
+    BwdPlan B;
+    const drot_status st = plan_bwd(n_problems, problems, bparams, &B);
+    if (st != DROT_OK) return st;
+    *bytes = B.total;
+    return DROT_OK;
+}
+
+drot_status drot_qr_batch_backward_f32(int n_problems, const drot_batch_problem *problems,
+                                       const float *c, const float *x, const float *grad_x,
+                                       const float *grad_obj, const drot_backward_params *bparams,
+                                       float *grad_c, void *workspace, size_t workspace_bytes,
+                                       void *stream, drot_backward_info *infos) {
+    BwdPlan B;
+    const drot_status st = plan_bwd(n_problems, problems, bparams, &B);
+    if (st != DROT_OK) return st;
+    return run_bwd(B, c, x, grad_x, grad_obj, grad_c, workspace, workspace_bytes,
+                   static_cast<hipStream_t>(stream), infos);
 }
 
 }  // extern "C"

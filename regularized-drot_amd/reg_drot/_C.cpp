@@ -121,8 +121,62 @@ torch::Tensor quadratic_drot(torch::Tensor c, torch::Tensor p, torch::Tensor q, 
 py::tuple quadratic_drot_ex(torch::Tensor c, torch::Tensor p, torch::Tensor q, float rho,
                             float r_weight, int max_iters, float eps) {
     check_shapes(c, p, q);
-    auto out = qr_impl(c, p, q, qr_params(p, q, rho, r_weight, max_iters, eps));
-    return py::make_tuple(out.first, info_dict(out.second));
+    const drot_qr_params prm = qr_params(p, q, rho, r_weight, max_iters, eps);
+    auto out = qr_impl(c, p, q, prm);
+    py::dict d = info_dict(out.second);
+    d["step_size"] = prm.step_size;
+    d["scale"] = prm.scale;
+    return py::make_tuple(out.first, d);
+}
+
+// dL/dC of a plan returned by quadratic_drot for the same (c, p, q, rho,
+// r_weight): implicit differentiation on the plan's support (no counterpart in
+// the reference, whose binding returns a tensor without gradient,
+// pytorch/src/qr_drot_cuda_kernel.cu:99).  grad_obj: optional 0-d/1-element
+// tensor on c's device, the gradient of the regularized objective.
+py::tuple quadratic_drot_backward(torch::Tensor c, torch::Tensor p, torch::Tensor q,
+                                  torch::Tensor x, torch::Tensor grad_x,
+                                  c10::optional<torch::Tensor> grad_obj, float rho,
+                                  float r_weight, float support_tol, int max_cg_iters,
+                                  double cg_tol) {
+    check_shapes(c, p, q);
+    CHECK_INPUT(x);
+    CHECK_INPUT(grad_x);
+    TORCH_CHECK(x.sizes() == c.sizes() && grad_x.sizes() == c.sizes(),
+                "x and grad_x must have the shape of C");
+    TORCH_CHECK(x.device() == c.device() && grad_x.device() == c.device(),
+                "x and grad_x must be on C's device");
+    torch::Tensor gobj;
+    if (grad_obj.has_value() && grad_obj->defined()) {
+        gobj = grad_obj->to(c.device(), torch::kFloat32).reshape({-1}).contiguous();
+        TORCH_CHECK(gobj.numel() == 1, "grad_obj must hold one value");
+    }
+    const int n_rows = (int)p.numel(), n_cols = (int)q.numel();
+    const drot_qr_params prm = qr_params(p, q, rho, r_weight, 0, 0.0f);
+    drot_backward_params bp{};
+    bp.support_tol = support_tol;
+    bp.max_cg_iters = max_cg_iters;
+    bp.cg_tol = cg_tol;
+    const c10::hip::HIPGuardMasqueradingAsCUDA guard(c.device());
+    size_t ws_bytes = 0;
+    check_status(drot_qr_backward_workspace_size(n_rows, n_cols, &prm, &bp, &ws_bytes),
+                 "drot_qr_backward_workspace_size");
+    auto ws = torch::empty({(int64_t)ws_bytes}, c.options().dtype(torch::kUInt8));
+    auto grad_c = torch::empty_like(c);
+    hipStream_t stream = c10::hip::getCurrentHIPStreamMasqueradingAsCUDA(c.device().index()).stream();
+    drot_backward_info info{};
+    check_status(drot_qr_backward_f32(c.data_ptr<float>(), x.data_ptr<float>(),
+                                      grad_x.data_ptr<float>(),
+                                      gobj.defined() ? gobj.data_ptr<float>() : nullptr, n_rows,
+                                      n_cols, &prm, &bp, grad_c.data_ptr<float>(), ws.data_ptr(),
+                                      ws_bytes, (void *)stream, &info),
+                 "drot_qr_backward_f32");
+    py::dict d;
+    d["nnz"] = info.nnz;
+    d["cg_iters"] = info.cg_iters;
+    d["cg_converged"] = (bool)info.converged;
+    d["cg_residual"] = info.cg_residual;
+    return py::make_tuple(grad_c, d);
 }
 
 // reg_drot.group_lasso_drot(c, p, q, n_class, rho, r_weight, max_iters, eps)
@@ -149,5 +203,11 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
           "Quadratic Regularized DROT, returning (x, info)");
     m.def("group_lasso_drot_ex", &group_lasso_drot_ex,
           "Group-Lasso Regularized DROT, returning (x, info)");
+    m.def("quadratic_drot_backward", &quadratic_drot_backward,
+          "Implicit-differentiation gradient dL/dC of a quadratic DROT plan, returning "
+          "(grad_c, info)",
+          py::arg("c"), py::arg("p"), py::arg("q"), py::arg("x"), py::arg("grad_x"),
+          py::arg("grad_obj"), py::arg("rho"), py::arg("r_weight"), py::arg("support_tol") = 0.0f,
+          py::arg("max_cg_iters") = 0, py::arg("cg_tol") = 0.0);
     m.attr("abi_version") = drot_abi_version();
 }

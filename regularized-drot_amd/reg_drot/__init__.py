@@ -18,6 +18,11 @@ fallback: importing this module fails if the native extension is not built.
 Extras that the reference computes but discards:
     x, info = reg_drot.quadratic_drot_ex(...)    # info: n_iter, objective, residual
     x, info = reg_drot.group_lasso_drot_ex(...)
+
+Additions with no reference counterpart (the reference has no backward and no
+batched path): reg_drot.quadratic_drot_diff / quadratic_drot_batch_diff
+(autograd through the implicit-differentiation backward, reg_drot.autograd)
+and reg_drot.batch (many problems per solve).
 """
 from __future__ import annotations
 
@@ -36,4 +41,7 @@ group_lasso_drot = _C.group_lasso_drot
 quadratic_drot_ex = _C.quadratic_drot_ex
 group_lasso_drot_ex = _C.group_lasso_drot_ex
 
-__all__ = ["quadratic_drot", "group_lasso_drot", "quadratic_drot_ex", "group_lasso_drot_ex"]
+from .autograd import quadratic_drot_batch_diff, quadratic_drot_diff  # noqa: E402
+
+__all__ = ["quadratic_drot", "group_lasso_drot", "quadratic_drot_ex", "group_lasso_drot_ex",
+           "quadratic_drot_diff", "quadratic_drot_batch_diff"]

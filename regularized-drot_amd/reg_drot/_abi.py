@@ -27,6 +27,8 @@ EXPORTS = (
     "drot_qr_wrapper_f32", "drot_glr_wrapper_f32", "drot_free",
     "drot_status_string", "drot_last_error", "drot_abi_version", "drot_set_timing",
     "drot_batch_layout", "drot_qr_batch_workspace_size", "drot_qr_batch_solve_f32",
+    "drot_qr_backward_workspace_size", "drot_qr_backward_f32",
+    "drot_qr_batch_backward_workspace_size", "drot_qr_batch_backward_f32",
 )
 
 DROT_OK = 0
@@ -62,6 +64,19 @@ class Result(ctypes.Structure):
         d["pass_ms"] = list(self.pass_ms)
         d["pass_count"] = list(self.pass_count)
         return d
+
+
+class BackwardParams(ctypes.Structure):
+    _fields_ = [("support_tol", ctypes.c_float), ("max_cg_iters", ctypes.c_int),
+                ("cg_tol", ctypes.c_double), ("nnz_capacity", ctypes.c_longlong)]
+
+
+class BackwardInfo(ctypes.Structure):
+    _fields_ = [("nnz", ctypes.c_longlong), ("cg_iters", ctypes.c_int),
+                ("converged", ctypes.c_int), ("cg_residual", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class DrotError(RuntimeError):
@@ -120,7 +135,16 @@ def load(path: str = LIB_PATH):
     lib.drot_qr_batch_workspace_size.argtypes = [c_int, P(BatchProblem), P(c_size)]
     lib.drot_qr_batch_solve_f32.argtypes = [c_int, P(BatchProblem), vp, vp, vp, vp, vp, c_size,
                                             vp, P(Result)]
-    for name in ("drot_qr_params_from_rho", "drot_qr_params_from_step",
+    BP, BI = P(BackwardParams), P(BackwardInfo)
+    lib.drot_qr_backward_workspace_size.argtypes = [c_int, c_int, P(QrParams), BP, P(c_size)]
+    lib.drot_qr_backward_f32.argtypes = [vp, vp, vp, vp, c_int, c_int, P(QrParams), BP, vp, vp,
+                                         c_size, vp, BI]
+    lib.drot_qr_batch_backward_workspace_size.argtypes = [c_int, P(BatchProblem), BP, P(c_size)]
+    lib.drot_qr_batch_backward_f32.argtypes = [c_int, P(BatchProblem), vp, vp, vp, vp, BP, vp,
+                                               vp, c_size, vp, BI]
+    for name in ("drot_qr_backward_workspace_size", "drot_qr_backward_f32",
+                 "drot_qr_batch_backward_workspace_size", "drot_qr_batch_backward_f32",
+                 "drot_qr_params_from_rho", "drot_qr_params_from_step",
                  "drot_glr_params_from_rho", "drot_glr_params_from_step",
                  "drot_qr_workspace_size", "drot_glr_workspace_size",
                  "drot_qr_solve_f32", "drot_glr_solve_f32", "drot_export_duals_f32",
@@ -234,3 +258,48 @@ def batch_solve(probs, c, p, q, x, ws, ws_bytes, stream):
     check(load().drot_qr_batch_solve_f32(n, probs, c, p, q, x, ws, ws_bytes, stream, res),
           "drot_qr_batch_solve_f32")
     return list(res)
+
+
+def backward_params(support_tol=0.0, max_cg_iters=0, cg_tol=0.0, nnz_capacity=0) -> BackwardParams:
+    return BackwardParams(float(support_tol), int(max_cg_iters), float(cg_tol), int(nnz_capacity))
+
+
+def qr_backward_workspace_size(n_rows, n_cols, prm: QrParams, bp: BackwardParams = None) -> int:
+    b = ctypes.c_size_t()
+    check(load().drot_qr_backward_workspace_size(n_rows, n_cols, ctypes.byref(prm),
+                                                 ctypes.byref(bp) if bp else None,
+                                                 ctypes.byref(b)),
+          "drot_qr_backward_workspace_size")
+    return b.value
+
+
+def qr_backward(c, x, grad_x, grad_obj, n_rows, n_cols, prm: QrParams, bp, grad_c, ws, ws_bytes,
+                stream) -> BackwardInfo:
+    """Device addresses for c/x/grad_x/grad_obj (0 = none)/grad_c/ws."""
+    info = BackwardInfo()
+    check(load().drot_qr_backward_f32(c, x, grad_x, grad_obj or None, n_rows, n_cols,
+                                      ctypes.byref(prm), ctypes.byref(bp) if bp else None,
+                                      grad_c, ws, ws_bytes, stream, ctypes.byref(info)),
+          "drot_qr_backward_f32")
+    return info
+
+
+def batch_backward_workspace_size(probs, bp: BackwardParams = None) -> int:
+    b = ctypes.c_size_t()
+    check(load().drot_qr_batch_backward_workspace_size(len(probs), probs,
+                                                       ctypes.byref(bp) if bp else None,
+                                                       ctypes.byref(b)),
+          "drot_qr_batch_backward_workspace_size")
+    return b.value
+
+
+def batch_backward(probs, c, x, grad_x, grad_obj, bp, grad_c, ws, ws_bytes, stream):
+    """Device addresses; grad_obj: device float array of len(probs) or 0.  Returns
+    a list of BackwardInfo."""
+    n = len(probs)
+    infos = (BackwardInfo * n)()
+    check(load().drot_qr_batch_backward_f32(n, probs, c, x, grad_x, grad_obj or None,
+                                            ctypes.byref(bp) if bp else None, grad_c, ws,
+                                            ws_bytes, stream, infos),
+          "drot_qr_batch_backward_f32")
+    return list(infos)

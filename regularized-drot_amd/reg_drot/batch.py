@@ -36,11 +36,14 @@ class PackedBatch:
                                 self.q.data_ptr(), self.x.data_ptr(), self.ws.data_ptr(),
                                 self.ws.numel(), s.cuda_stream)
 
-    def plan(self, i: int) -> torch.Tensor:
+    def block(self, t: torch.Tensor, i: int) -> torch.Tensor:
+        """The [n_cols, n_rows] view of problem i in a packed matrix buffer t."""
         pr = self.probs[i]
         n, m = pr.n_rows, pr.n_cols
-        block = self.x[pr.mat_offset: pr.mat_offset + pr.ld * m].view(m, pr.ld)
-        return block[:, :n]
+        return t[pr.mat_offset: pr.mat_offset + pr.ld * m].view(m, pr.ld)[:, :n]
+
+    def plan(self, i: int) -> torch.Tensor:
+        return self.block(self.x, i)
 
 
 def pack(cs, ps, qs, rho, r_weight, max_iters, eps) -> PackedBatch:
