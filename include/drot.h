@@ -231,6 +231,45 @@ drot_status drot_qr_batch_backward_f32(int n_problems, const drot_batch_problem 
                                        float *grad_c, void *workspace, size_t workspace_bytes,
                                        void *stream, drot_backward_info *infos);
 
+/* ------------------------------------------------------- cost matrices */
+
+/* Squared-Euclidean cost between point clouds, written in the solver layout:
+ * C[j * ld + i] = ||src_i - tgt_j||^2 for source point i < n_rows (the p side)
+ * and target point j < n_cols (the q side), optionally divided by max C.  The
+ * reference computes this on the host with POT before the solve
+ * (benchmark/generate_ot.py:22-23, benchmark/generate_da.py:40-41: ot.dist
+ * and C / C.max()); here it is a device step that can write straight into a
+ * packed batch (mat_offset, ld as in drot_batch_problem). */
+typedef struct drot_cost_problem {
+    int n_rows, n_cols, ld;
+    long long mat_offset;               /* float offset of C */
+    long long src_offset, tgt_offset;   /* first point (row of the [*, dim] buffers) */
+} drot_cost_problem;
+
+/* dim <= 64. */
+drot_status drot_sqdist_workspace_size(int n_problems, const drot_cost_problem *problems, int dim,
+                                       size_t *bytes);
+
+/* src, tgt: row-major [points, dim] device buffers.  normalize != 0 divides by
+ * max C (1 if every distance is 0) and then needs the device outputs
+ * scale[n_problems] (max C) and argmax[2 n_problems] (i*, j*; smallest
+ * j*n_rows+i on ties), which the backward reads; they may be NULL otherwise.
+ * Padding rows n_rows..ld-1 of a problem's last row tile are set to 0.
+ * Asynchronous on `stream`. */
+drot_status drot_sqdist_f32(int n_problems, const drot_cost_problem *problems, int dim,
+                            const float *src, const float *tgt, int normalize, float *c,
+                            float *scale, int *argmax, void *workspace, size_t workspace_bytes,
+                            void *stream);
+
+/* Gradient of a loss through C to the points: grad_src, grad_tgt receive
+ * dL/dsrc, dL/dtgt (same layout as src, tgt) for grad_c = dL/dC, including
+ * the derivative of max C through (i*, j*) when normalize != 0.  Asynchronous. */
+drot_status drot_sqdist_backward_f32(int n_problems, const drot_cost_problem *problems, int dim,
+                                     const float *src, const float *tgt, const float *grad_c,
+                                     int normalize, const float *scale, const int *argmax,
+                                     float *grad_src, float *grad_tgt, void *workspace,
+                                     size_t workspace_bytes, void *stream);
+
 /* ---------------------------------------------------------------- host API */
 
 /* Host-pointer wrapper with the argument order of

@@ -34,7 +34,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["drot_kernels.hip", "drot_qr.hip", "drot_batch.hip", "drot_backward.hip",
-           "drot_solver.hip"]
+           "drot_cost.hip", "drot_solver.hip"]
 
 
 def _run(cmd, quiet_filter=()):

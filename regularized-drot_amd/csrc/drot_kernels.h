@@ -203,4 +203,45 @@ hipError_t launch_bwd_count(const BwdArgs &a, hipStream_t s);
 hipError_t launch_bwd_fill(const BwdArgs &a, hipStream_t s);
 hipError_t launch_bwd_cg(const BwdArgs &a, hipStream_t s);
 
+// ----------------------------------------------------------- cost matrices
+//
+// C_ij = ||src_i - tgt_j||^2 (optionally / max C) for point clouds stored
+// row-major [n_points, dim]; C in the solver layout (i contiguous).  Tiles of
+// kCostTile source x kCostTile target points (drot_cost.hip).
+
+constexpr int kCostTile = 64;
+constexpr int kCostMaxDim = 64;
+
+struct CostProblem {
+    long long mat_off;           // float offset of C (leading dimension ld)
+    long long src_off, tgt_off;  // first point of the problem in src / tgt
+    long long xpart_off;         // fp64 partials: n_jt * n_rows * (dim + 1)
+    long long ypart_off;         // fp64 partials: n_it * n_cols * (dim + 1)
+    int n_rows, n_cols, ld;      // n_rows source points, n_cols target points
+    int tile_begin, n_it, n_jt;
+};
+
+struct CostTile {
+    int prob, it, jt, pad;
+};
+
+struct CostArgs {
+    const float *src, *tgt;
+    float *c;
+    const float *grad_c;
+    float *grad_src, *grad_tgt;
+    const CostProblem *probs;
+    const CostTile *tiles;
+    float *tmax;                 // per tile: largest D
+    int *targ;                   // per tile: its index j * n_rows + i (smallest on ties)
+    float *scale;                // per problem: max D (1 when normalisation is off or max D = 0)
+    int *argmax;                 // per problem: i*, j*
+    double *xpart, *ypart;
+    double *tdot;                // per tile: sum of grad_c * C
+    int dim, normalize, n_tiles, n_probs, max_points;
+};
+
+hipError_t launch_cost_forward(const CostArgs &a, hipStream_t s);
+hipError_t launch_cost_backward(const CostArgs &a, hipStream_t s);
+
 }  // namespace drot

@@ -1254,6 +1254,134 @@ drot_batch_problem single_problem(int n_rows, int n_cols, const drot_qr_params *
     return d;
 }
 
+
+// ---------------------------------------------------------- cost matrices
+
+// Asynchronous upload of small host tables: two pinned buffers used in turn,
+// each reused only after the copy that last read it has completed.
+struct UploadRing {
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool pending[2] = {false, false};
+    size_t cap = 0;
+    int next = 0;
+    int device = -1;
+};
+thread_local UploadRing g_ring;
+
+drot_status upload_async(void *dst, const void *src, size_t bytes, hipStream_t stream) {
+    int dev = 0;
+    DROT_HIP_CHECK(hipGetDevice(&dev));
+    UploadRing &R = g_ring;
+    if (R.device != dev || R.cap < bytes) {
+        for (int h = 0; h < 2; h++)  // old buffers stay allocated (process lifetime)
+            if (R.pending[h]) DROT_HIP_CHECK(hipEventSynchronize(R.ev[h]));
+        R = UploadRing();
+        R.cap = std::max<size_t>(bytes, 1 << 16);
+        for (int h = 0; h < 2; h++) {
+            DROT_HIP_CHECK(hipHostMalloc(&R.buf[h], R.cap, hipHostMallocDefault));
+            DROT_HIP_CHECK(hipEventCreateWithFlags(&R.ev[h], hipEventDisableTiming));
+        }
+        R.device = dev;
+    }
+    const int h = R.next;
+    R.next ^= 1;
+    if (R.pending[h]) DROT_HIP_CHECK(hipEventSynchronize(R.ev[h]));
+    memcpy(R.buf[h], src, bytes);
+    DROT_HIP_CHECK(hipMemcpyAsync(dst, R.buf[h], bytes, hipMemcpyHostToDevice, stream));
+    DROT_HIP_CHECK(hipEventRecord(R.ev[h], stream));
+    R.pending[h] = true;
+    return DROT_OK;
+}
+
+struct CostPlan {
+    std::vector<CostProblem> probs;
+    std::vector<CostTile> tiles;
+    int max_points = 0;
+    size_t off_probs = 0, off_tiles = 0, off_tmax = 0, off_targ = 0, off_xpart = 0;
+    size_t off_ypart = 0, off_tdot = 0, table_bytes = 0, total = 0;
+};
+
+drot_status plan_cost(int n, const drot_cost_problem *in, int dim, CostPlan *out) {
+    if (n <= 0 || !in) return fail(DROT_ERR_INVALID_ARGUMENT, "empty problem list");
+    if (dim < 1) return fail(DROT_ERR_INVALID_ARGUMENT, "dim must be >= 1");
+    if (dim > kCostMaxDim)
+        return fail(DROT_ERR_UNSUPPORTED, "dim " + std::to_string(dim) + " > " +
+                                              std::to_string(kCostMaxDim));
+    CostPlan B;
+    long long xpart = 0, ypart = 0;
+    for (int i = 0; i < n; i++) {
+        const drot_cost_problem &d = in[i];
+        const std::string who = "problem " + std::to_string(i);
+        if (d.n_rows <= 0 || d.n_cols <= 0) return fail(DROT_ERR_INVALID_ARGUMENT, who + ": bad shape");
+        if (d.ld < d.n_rows || d.mat_offset < 0 || d.src_offset < 0 || d.tgt_offset < 0)
+            return fail(DROT_ERR_INVALID_ARGUMENT, who + ": ld must be >= n_rows, offsets >= 0");
+        if ((long long)d.n_rows * d.n_cols >= (1LL << 31))
+            return fail(DROT_ERR_UNSUPPORTED, who + ": more than 2^31 - 1 elements");
+        CostProblem P{};
+        P.mat_off = d.mat_offset;
+        P.src_off = d.src_offset;
+        P.tgt_off = d.tgt_offset;
+        P.n_rows = d.n_rows;
+        P.n_cols = d.n_cols;
+        P.ld = d.ld;
+        P.n_it = ceil_div(d.n_rows, kCostTile);
+        P.n_jt = ceil_div(d.n_cols, kCostTile);
+        P.tile_begin = (int)B.tiles.size();
+        P.xpart_off = xpart;
+        P.ypart_off = ypart;
+        xpart += (long long)P.n_jt * d.n_rows * (dim + 1);
+        ypart += (long long)P.n_it * d.n_cols * (dim + 1);
+        for (int jt = 0; jt < P.n_jt; jt++)
+            for (int it = 0; it < P.n_it; it++) B.tiles.push_back(CostTile{i, it, jt, 0});
+        if (B.tiles.size() >= (size_t)(1u << 31)) return fail(DROT_ERR_UNSUPPORTED, "too many tiles");
+        B.max_points = std::max(B.max_points, d.n_rows + d.n_cols);
+        B.probs.push_back(P);
+    }
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = align_up(off + bytes, 256);
+        return o;
+    };
+    B.off_probs = take(sizeof(CostProblem) * B.probs.size());
+    B.off_tiles = take(sizeof(CostTile) * B.tiles.size());
+    B.table_bytes = off;
+    B.off_tmax = take(sizeof(float) * B.tiles.size());
+    B.off_targ = take(sizeof(int) * B.tiles.size());
+    B.off_tdot = take(sizeof(double) * B.tiles.size());
+    B.off_xpart = take(sizeof(double) * (size_t)xpart);
+    B.off_ypart = take(sizeof(double) * (size_t)ypart);
+    B.total = off;
+    *out = std::move(B);
+    return DROT_OK;
+}
+
+drot_status cost_args(const CostPlan &B, int dim, void *ws, size_t ws_bytes, hipStream_t stream,
+                      CostArgs *A) {
+    if (!ws) return fail(DROT_ERR_INVALID_ARGUMENT, "workspace is NULL");
+    if (ws_bytes < B.total)
+        return fail(DROT_ERR_WORKSPACE_TOO_SMALL, "workspace needs " + std::to_string(B.total) + " bytes");
+    char *base = static_cast<char *>(ws);
+    std::vector<char> tab(B.table_bytes, 0);
+    memcpy(tab.data() + B.off_probs, B.probs.data(), sizeof(CostProblem) * B.probs.size());
+    memcpy(tab.data() + B.off_tiles, B.tiles.data(), sizeof(CostTile) * B.tiles.size());
+    const drot_status st = upload_async(base, tab.data(), B.table_bytes, stream);
+    if (st != DROT_OK) return st;
+    A->probs = reinterpret_cast<const CostProblem *>(base + B.off_probs);
+    A->tiles = reinterpret_cast<const CostTile *>(base + B.off_tiles);
+    A->tmax = reinterpret_cast<float *>(base + B.off_tmax);
+    A->targ = reinterpret_cast<int *>(base + B.off_targ);
+    A->tdot = reinterpret_cast<double *>(base + B.off_tdot);
+    A->xpart = reinterpret_cast<double *>(base + B.off_xpart);
+    A->ypart = reinterpret_cast<double *>(base + B.off_ypart);
+    A->dim = dim;
+    A->n_tiles = (int)B.tiles.size();
+    A->n_probs = (int)B.probs.size();
+    A->max_points = B.max_points;
+    return DROT_OK;
+}
+
 }  // namespace
 }  // namespace drot
 
@@ -1359,6 +1487,68 @@ drot_status drot_qr_batch_backward_f32(int n_problems, const drot_batch_problem 
     if (st != DROT_OK) return st;
     return run_bwd(B, c, x, grad_x, grad_obj, grad_c, workspace, workspace_bytes,
                    static_cast<hipStream_t>(stream), infos);
+}
+
+drot_status drot_sqdist_workspace_size(int n_problems, const drot_cost_problem *problems, int dim,
+                                       size_t *bytes) {
+    if (!bytes) return fail(DROT_ERR_INVALID_ARGUMENT, "bytes is NULL");
+    CostPlan B;
+    const drot_status st = plan_cost(n_problems, problems, dim, &B);
+    if (st != DROT_OK) return st;
+    *bytes = B.total;
+    return DROT_OK;
+}
+
+drot_status drot_sqdist_f32(int n_problems, const drot_cost_problem *problems, int dim,
+                            const float *src, const float *tgt, int normalize, float *c,
+                            float *scale, int *argmax, void *workspace, size_t workspace_bytes,
+                            void *stream) {
+    if (!src || !tgt || !c) return fail(DROT_ERR_INVALID_ARGUMENT, "null pointer argument");
+    if (normalize && (!scale || !argmax))
+        return fail(DROT_ERR_INVALID_ARGUMENT, "normalize needs the scale and argmax outputs");
+    CostPlan B;
+    drot_status st = plan_cost(n_problems, problems, dim, &B);
+    if (st != DROT_OK) return st;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    CostArgs A{};
+    st = cost_args(B, dim, workspace, workspace_bytes, s, &A);
+    if (st != DROT_OK) return st;
+    A.src = src;
+    A.tgt = tgt;
+    A.c = c;
+    A.scale = scale;
+    A.argmax = argmax;
+    A.normalize = normalize ? 1 : 0;
+    DROT_HIP_CHECK(launch_cost_forward(A, s));
+    return DROT_OK;
+}
+
+drot_status drot_sqdist_backward_f32(int n_problems, const drot_cost_problem *problems, int dim,
+                                     const float *src, const float *tgt, const float *grad_c,
+                                     int normalize, const float *scale, const int *argmax,
+                                     float *grad_src, float *grad_tgt, void *workspace,
+                                     size_t workspace_bytes, void *stream) {
+    if (!src || !tgt || !grad_c || !grad_src || !grad_tgt)
+        return fail(DROT_ERR_INVALID_ARGUMENT, "null pointer argument");
+    if (normalize && (!scale || !argmax))
+        return fail(DROT_ERR_INVALID_ARGUMENT, "normalize needs scale and argmax from the forward");
+    CostPlan B;
+    drot_status st = plan_cost(n_problems, problems, dim, &B);
+    if (st != DROT_OK) return st;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    CostArgs A{};
+    st = cost_args(B, dim, workspace, workspace_bytes, s, &A);
+    if (st != DROT_OK) return st;
+    A.src = src;
+    A.tgt = tgt;
+    A.grad_c = grad_c;
+    A.grad_src = grad_src;
+    A.grad_tgt = grad_tgt;
+    A.scale = const_cast<float *>(scale);
+    A.argmax = const_cast<int *>(argmax);
+    A.normalize = normalize ? 1 : 0;
+    DROT_HIP_CHECK(launch_cost_backward(A, s));
+    return DROT_OK;
 }
 
 }  // extern "C"

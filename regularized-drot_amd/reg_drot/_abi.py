@@ -29,6 +29,7 @@ EXPORTS = (
     "drot_batch_layout", "drot_qr_batch_workspace_size", "drot_qr_batch_solve_f32",
     "drot_qr_backward_workspace_size", "drot_qr_backward_f32",
     "drot_qr_batch_backward_workspace_size", "drot_qr_batch_backward_f32",
+    "drot_sqdist_workspace_size", "drot_sqdist_f32", "drot_sqdist_backward_f32",
 )
 
 DROT_OK = 0
@@ -77,6 +78,12 @@ class BackwardInfo(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CostProblem(ctypes.Structure):
+    _fields_ = [("n_rows", ctypes.c_int), ("n_cols", ctypes.c_int), ("ld", ctypes.c_int),
+                ("mat_offset", ctypes.c_longlong), ("src_offset", ctypes.c_longlong),
+                ("tgt_offset", ctypes.c_longlong)]
 
 
 class DrotError(RuntimeError):
@@ -142,7 +149,13 @@ def load(path: str = LIB_PATH):
     lib.drot_qr_batch_backward_workspace_size.argtypes = [c_int, P(BatchProblem), BP, P(c_size)]
     lib.drot_qr_batch_backward_f32.argtypes = [c_int, P(BatchProblem), vp, vp, vp, vp, BP, vp,
                                                vp, c_size, vp, BI]
-    for name in ("drot_qr_backward_workspace_size", "drot_qr_backward_f32",
+    CP = P(CostProblem)
+    lib.drot_sqdist_workspace_size.argtypes = [c_int, CP, c_int, P(c_size)]
+    lib.drot_sqdist_f32.argtypes = [c_int, CP, c_int, vp, vp, c_int, vp, vp, vp, vp, c_size, vp]
+    lib.drot_sqdist_backward_f32.argtypes = [c_int, CP, c_int, vp, vp, vp, c_int, vp, vp, vp, vp,
+                                             vp, c_size, vp]
+    for name in ("drot_sqdist_workspace_size", "drot_sqdist_f32", "drot_sqdist_backward_f32",
+                 "drot_qr_backward_workspace_size", "drot_qr_backward_f32",
                  "drot_qr_batch_backward_workspace_size", "drot_qr_batch_backward_f32",
                  "drot_qr_params_from_rho", "drot_qr_params_from_step",
                  "drot_glr_params_from_rho", "drot_glr_params_from_step",
@@ -303,3 +316,41 @@ def batch_backward(probs, c, x, grad_x, grad_obj, bp, grad_c, ws, ws_bytes, stre
                                             ws_bytes, stream, infos),
           "drot_qr_batch_backward_f32")
     return list(infos)
+
+
+def cost_problems(shapes, lds=None, mat_offsets=None):
+    """ctypes array of CostProblem for [(n_rows, n_cols)]: points of problem i
+    follow those of problem i-1 in the packed src / tgt buffers; C blocks are
+    dense (ld = n_rows) unless lds / mat_offsets are given."""
+    n = len(shapes)
+    probs = (CostProblem * n)()
+    so = to = mo = 0
+    for i, (nr, nc) in enumerate(shapes):
+        ld = int(lds[i]) if lds is not None else int(nr)
+        off = int(mat_offsets[i]) if mat_offsets is not None else mo
+        probs[i] = CostProblem(int(nr), int(nc), ld, off, so, to)
+        so += nr
+        to += nc
+        mo = off + ld * nc
+    return probs
+
+
+def sqdist_workspace_size(probs, dim) -> int:
+    b = ctypes.c_size_t()
+    check(load().drot_sqdist_workspace_size(len(probs), probs, int(dim), ctypes.byref(b)),
+          "drot_sqdist_workspace_size")
+    return b.value
+
+
+def sqdist(probs, dim, src, tgt, normalize, c, scale, argmax, ws, ws_bytes, stream):
+    check(load().drot_sqdist_f32(len(probs), probs, int(dim), src, tgt, int(bool(normalize)), c,
+                                 scale or None, argmax or None, ws, ws_bytes, stream),
+          "drot_sqdist_f32")
+
+
+def sqdist_backward(probs, dim, src, tgt, grad_c, normalize, scale, argmax, grad_src, grad_tgt,
+                    ws, ws_bytes, stream):
+    check(load().drot_sqdist_backward_f32(len(probs), probs, int(dim), src, tgt, grad_c,
+                                          int(bool(normalize)), scale or None, argmax or None,
+                                          grad_src, grad_tgt, ws, ws_bytes, stream),
+          "drot_sqdist_backward_f32")

@@ -107,6 +107,13 @@ def pack_costs(cs, pb) -> torch.Tensor:
     return torch.cat(parts)
 
 
+def batch_solve_diff(cp, pb, *, support_tol=0.0, max_cg_iters=0, cg_tol=0.0, nnz_capacity=0):
+    """Differentiable solve of the packed costs cp (a flat tensor in pb's layout,
+    e.g. from reg_drot.cost.sqdist_packed): returns (packed plans, objectives)."""
+    bp = _abi.backward_params(support_tol, max_cg_iters, cg_tol, nnz_capacity)
+    return _BatchFunction.apply(cp, pb, bp)
+
+
 def quadratic_drot_batch_diff(cs, ps, qs, rho, r_weight, max_iters, eps, *, support_tol=0.0,
                               max_cg_iters=0, cg_tol=0.0, nnz_capacity=0, packed=None):
     """Differentiable batched solve: returns (plans, objectives[n] fp64).
@@ -114,10 +121,9 @@ def quadratic_drot_batch_diff(cs, ps, qs, rho, r_weight, max_iters, eps, *, supp
     Gradients flow to every cs[i] through one batched backward launch sequence
     (drot_qr_batch_backward_f32).  `packed` may pass a PackedBatch from an
     earlier call with the same shapes and parameters, to reuse its buffers."""
-    pb = packed if packed is not None else _batch.pack(
-        [c.detach() for c in cs], ps, qs, rho, r_weight, max_iters, eps)
+    pb = packed if packed is not None else _batch.layout(ps, qs, rho, r_weight, max_iters, eps)
     cp = pack_costs(cs, pb)
-    bp = _abi.backward_params(support_tol, max_cg_iters, cg_tol, nnz_capacity)
-    xp, objs = _BatchFunction.apply(cp, pb, bp)
+    xp, objs = batch_solve_diff(cp, pb, support_tol=support_tol, max_cg_iters=max_cg_iters,
+                                cg_tol=cg_tol, nnz_capacity=nnz_capacity)
     plans = [pb.block(xp, i) for i in range(len(cs))]
     return plans, objs

@@ -46,36 +46,54 @@ class PackedBatch:
         return self.block(self.x, i)
 
 
-def pack(cs, ps, qs, rho, r_weight, max_iters, eps) -> PackedBatch:
-    if not (len(cs) == len(ps) == len(qs)) or len(cs) == 0:
-        raise RuntimeError("cs, ps and qs must be non-empty lists of the same length")
-    dev = cs[0].device
-    for i, (c, p, q) in enumerate(zip(cs, ps, qs)):
-        if not (c.is_cuda and p.is_cuda and q.is_cuda):
+def _check_marginals(ps, qs):
+    for i, (p, q) in enumerate(zip(ps, qs)):
+        if not (p.is_cuda and q.is_cuda):
             raise RuntimeError(f"problem {i}: tensors must be CUDA tensors")
-        if c.dtype != torch.float32 or p.dtype != torch.float32 or q.dtype != torch.float32:
+        if p.dtype != torch.float32 or q.dtype != torch.float32:
             raise RuntimeError(f"problem {i}: tensors must be float32")
-        if c.dim() != 2 or c.shape[0] != q.numel() or c.shape[1] != p.numel():
-            raise RuntimeError(f"problem {i}: C must be [q.numel(), p.numel()]")
+
+
+def layout(ps, qs, rho, r_weight, max_iters, eps) -> PackedBatch:
+    """Packed buffers for problems with marginals ps[i], qs[i]; the cost buffer
+    is zero-filled (write it, e.g. with reg_drot.cost.sqdist_packed)."""
+    if len(ps) != len(qs) or len(ps) == 0:
+        raise RuntimeError("ps and qs must be non-empty lists of the same length")
+    _check_marginals(ps, qs)
+    dev = ps[0].device
     n_rows = [p.numel() for p in ps]
     n_cols = [q.numel() for q in qs]
     probs, mat_elems, p_elems, q_elems = _abi.batch_layout(n_rows, n_cols)
-    rws = list(r_weight) if isinstance(r_weight, (list, tuple)) else [r_weight] * len(cs)
-    for i in range(len(cs)):
+    rws = list(r_weight) if isinstance(r_weight, (list, tuple)) else [r_weight] * len(ps)
+    for i in range(len(ps)):
         probs[i].params = _abi.qr_params_from_rho(rho, float(rws[i]), n_rows[i], n_cols[i],
                                                   max_iters, eps)
     cp = torch.zeros(mat_elems, dtype=torch.float32, device=dev)
     pp = torch.empty(p_elems, dtype=torch.float32, device=dev)
     qp = torch.empty(q_elems, dtype=torch.float32, device=dev)
-    for i, (c, p, q) in enumerate(zip(cs, ps, qs)):
+    for i, (p, q) in enumerate(zip(ps, qs)):
         pr = probs[i]
-        cp[pr.mat_offset: pr.mat_offset + pr.ld * pr.n_cols].view(pr.n_cols, pr.ld)[
-            :, :pr.n_rows].copy_(c)
         pp[pr.p_offset: pr.p_offset + pr.n_rows].copy_(p)
         qp[pr.q_offset: pr.q_offset + pr.n_cols].copy_(q)
     x = torch.empty_like(cp)
     ws = torch.empty(_abi.batch_workspace_size(probs), dtype=torch.uint8, device=dev)
     return PackedBatch(probs, cp, pp, qp, x, ws, list(zip(n_rows, n_cols)))
+
+
+def pack(cs, ps, qs, rho, r_weight, max_iters, eps) -> PackedBatch:
+    if not (len(cs) == len(ps) == len(qs)) or len(cs) == 0:
+        raise RuntimeError("cs, ps and qs must be non-empty lists of the same length")
+    for i, (c, p, q) in enumerate(zip(cs, ps, qs)):
+        if not (c.is_cuda and p.is_cuda and q.is_cuda):
+            raise RuntimeError(f"problem {i}: tensors must be CUDA tensors")
+        if c.dtype != torch.float32:
+            raise RuntimeError(f"problem {i}: tensors must be float32")
+        if c.dim() != 2 or c.shape[0] != q.numel() or c.shape[1] != p.numel():
+            raise RuntimeError(f"problem {i}: C must be [q.numel(), p.numel()]")
+    pb = layout(ps, qs, rho, r_weight, max_iters, eps)
+    for i, c in enumerate(cs):
+        pb.block(pb.c, i).copy_(c)
+    return pb
 
 
 def quadratic_drot_batch(cs, ps, qs, rho, r_weight, max_iters, eps):
