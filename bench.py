@@ -14,7 +14,19 @@ solver's stream (library timing mode) over the timed steps.  cpu_baseline is
 the CPU oracle (a C port of the same iteration, OpenMP over all host threads)
 timed on a bounded sample of iterations of the same workload.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+The "batched" block carries the second half of the metric, forward+backward
+solves per second of batched QR-DROT against looping the single-problem solver
+on the same GPU, for two workloads:
+  differentiable: 64 problems of 512 x 512, C = normalised squared distances
+      between learnable N(0, I_16) point clouds, loss = sum(W o X*) + F with W
+      iid N(0,1) and F the regularized objective, r_weight 1e-3; one step =
+      cost kernel, solve to 1e-4, implicit backward, gradient to the points;
+  mixed: 256 problems with m, n ~ U[64, 2048], R^8 clouds, r_weight
+      log-uniform in [1e-4, 1e-2], same loss.
+The looped path runs cost, reg_drot.quadratic_drot_diff and backward problem by
+problem; both paths produce the same gradients (checked).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-batched]
 """
 from __future__ import annotations
 
@@ -46,6 +58,11 @@ def parse():
     ap.add_argument("--cpu-sample-s", type=float, default=15.0,
                     help="target seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched forward+backward comparison")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune the pass tile for the workload instead of loading the "
+                         "measured tile table")
     ap.add_argument("--profile-iters", type=int, default=0,
                     help="run one solve of exactly this many iterations and exit "
                          "(for rocprofv3 counter runs)")
@@ -99,6 +116,75 @@ def cpu_baseline(c, p, q, step, kappa, target_s):
             "us_per_iter": round(dt / (2 * pairs) * 1e6, 1)}
 
 
+def batched_fwd_bwd(dev, workload):
+    """One batched and one looped forward+backward pass over a workload; returns
+    times, solves/s and the largest gradient difference between the two."""
+    import numpy as np
+    import torch
+
+    import reg_drot
+    from reg_drot import autograd, batch, cost
+
+    rho, eps, max_iters = 2.0, 1e-4, 20000
+    if workload == "differentiable":
+        shapes, dim = [(512, 512)] * 64, 16
+        rws = [1e-3] * 64
+    else:
+        rng = np.random.default_rng(0)
+        shapes = [(int(a), int(b)) for a, b in rng.integers(64, 2049, size=(256, 2))]
+        dim = 8
+        rws = [float(v) for v in 10 ** rng.uniform(-4, -2, size=256)]
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    x0 = [torch.randn(n, dim, device=dev, generator=g) for n, _ in shapes]
+    y0 = [torch.randn(m, dim, device=dev, generator=g) for _, m in shapes]
+    w = [torch.randn(m, n, device=dev, generator=g) for n, m in shapes]
+    ps = [torch.full((n,), 1.0 / n, device=dev) for n, _ in shapes]
+    qs = [torch.full((m,), 1.0 / m, device=dev) for _, m in shapes]
+    pb = batch.layout(ps, qs, rho, rws, max_iters, eps)
+
+    def leaves():
+        return ([x.clone().requires_grad_(True) for x in x0],
+                [y.clone().requires_grad_(True) for y in y0])
+
+    def batched():
+        xs, ys = leaves()
+        xp, objs = autograd.batch_solve_diff(cost.sqdist_packed(xs, ys, pb), pb)
+        loss = objs.sum()
+        for i in range(len(shapes)):
+            loss = loss + (w[i] * pb.block(xp, i)).sum()
+        loss.backward()
+        return xs
+
+    def looped():
+        xs, ys = leaves()
+        for i in range(len(shapes)):
+            c = cost.sqdist(xs[i], ys[i])
+            x, f = reg_drot.quadratic_drot_diff(c, ps[i], qs[i], rho, rws[i], max_iters, eps,
+                                                return_objective=True)
+            ((w[i] * x).sum() + f).backward()
+        return xs
+
+    batched()
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    xb = batched()
+    torch.cuda.synchronize(dev)
+    tb = time.perf_counter() - t0
+    looped()
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    xl = looped()
+    torch.cuda.synchronize(dev)
+    tl = time.perf_counter() - t0
+    diff = max(float((a.grad - b.grad).abs().max() / b.grad.abs().max().clamp_min(1e-30))
+               for a, b in zip(xb, xl))
+    n = len(shapes)
+    return {"problems": n, "dim": dim, "batched_s": round(tb, 4), "looped_s": round(tl, 4),
+            "batched_solves_per_s": round(n / tb, 2), "looped_solves_per_s": round(n / tl, 2),
+            "speedup": round(tl / tb, 2), "grad_max_rel_diff": diff}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -110,7 +196,7 @@ def main():
     import numpy as np
     import torch
 
-    from reg_drot import _abi, problems
+    from reg_drot import _abi, problems, tune
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
@@ -123,6 +209,8 @@ def main():
         print(json.dumps({"copy_calibration_bytes_read": 4 * N_ROWS * N_COLS,
                           "copy_calibration_bytes_written": 4 * N_ROWS * N_COLS, "copies": 5}))
         return
+    if os.path.exists(tune.DEFAULT_TABLE):
+        tune.load(tune.DEFAULT_TABLE)
     c_np = problems.cloud_cost(N_ROWS, N_COLS, dim=16, seed=0)
     p_np, q_np = problems.uniform_marginals(N_ROWS, N_COLS)
     c = torch.from_numpy(c_np).to(dev)
@@ -133,6 +221,9 @@ def main():
     ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
     prm = _abi.qr_params_from_rho(RHO, R_WEIGHT, N_ROWS, N_COLS, MAX_ITERS, EPS)
     stream = torch.cuda.current_stream(dev).cuda_stream
+    if args.tune:
+        tune.autotune(c, p, q, RHO, R_WEIGHT)
+    tile = tune.tile_for(N_ROWS, N_COLS)
 
     def run_once():
         return _abi.qr_solve(c.data_ptr(), p.data_ptr(), q.data_ptr(), N_ROWS, N_COLS, prm,
@@ -199,7 +290,7 @@ def main():
                                "(warm start, rho=2, r_weight=1e-3, C=sqdist/max of "
                                "N(0,I_16) clouds, uniform marginals)",
                    "n_rows": N_ROWS, "n_cols": N_COLS, "r_weight": R_WEIGHT, "rho": RHO,
-                   "eps": EPS},
+                   "eps": EPS, "pass_tile": list(tile)},
         "time_to_tol_ms": round(ms_per_step, 4),
         "iterations": last.iterations_run,
         "n_iter_reference_convention": last.n_iter,
@@ -213,13 +304,20 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic_from_profiles(),
-            "kernel": "qr_pass_kernel<VEC=4,LPC=32,EVEN,ST=sc1,U=2> (12 B/element: read C, "
-                      "read X, write X)",
+            "kernel": "qr_pass_kernel<VEC=%d,LPC=%d,EVEN,ST=%d,U=%d>, %d column steps "
+                      "(12 B/element: read C, read X, write X)"
+                      % (tile[0], tile[1], tile[3], tile[4], tile[2]),
             "avg_us_even": round(even_avg_us, 3),
             "avg_us_odd": round(odd_avg_us, 3),
             "odd_pass_GBs": round(odd_gbs, 1),
         },
     }
+    if not args.no_batched:
+        out["batched"] = {
+            "metric": "forward+backward solves/s, batched vs looped single-problem solver",
+            "differentiable_64x512x512": batched_fwd_bwd(dev, "differentiable"),
+            "mixed_256_m_n_64_2048": batched_fwd_bwd(dev, "mixed"),
+        }
     if not args.no_cpu:
         s = np.float32(prm.step_size)
         k = np.float32(prm.scale)

@@ -231,6 +231,40 @@ drot_status drot_qr_batch_backward_f32(int n_problems, const drot_batch_problem 
                                        float *grad_c, void *workspace, size_t workspace_bytes,
                                        void *stream, drot_backward_info *infos);
 
+/* ---------------------------------------------------------------- tuning */
+
+/* Tile of the QR pass kernel: vec = floats per access (1 or 4), lanes_per_col
+ * (16, 32, 64) lanes share a column, col_steps column steps per block,
+ * store_policy 2 = write-through X stores, 0 = plain stores, unroll (2 or 4)
+ * column steps loaded per batch.  Replaces the reference's launch-size
+ * heuristic _q_get_work_size_update_x (core/drot_qr.hpp:236-239) and its
+ * launch macros (core/param_qr.hpp:4-24). */
+typedef struct drot_qr_tile {
+    int vec, lanes_per_col, col_steps, store_policy, unroll;
+} drot_qr_tile;
+
+/* The tile a solve of this shape will use (tuned, else the built-in choice). */
+drot_status drot_qr_tile_for(int n_rows, int n_cols, int vec4_ok, drot_qr_tile *out);
+
+/* Autotuner: times every candidate tile on this problem (k/5..k iterations
+ * from the warm start, k = max(iters, about 4 ms of work); x is scratch)
+ * and records the fastest for (n_rows, n_cols) in the process-wide tile cache,
+ * which later solves of that shape use.  The tuned tile can change
+ * drot_qr_workspace_size: query it after tuning.  Workspace:
+ * drot_qr_autotune_workspace_size.  Synchronous on `stream`. */
+drot_status drot_qr_autotune_workspace_size(int n_rows, int n_cols, size_t *bytes);
+drot_status drot_qr_autotune_f32(const float *c, const float *p, const float *q, int n_rows,
+                                 int n_cols, const drot_qr_params *params, float *x,
+                                 void *workspace, size_t workspace_bytes, void *stream,
+                                 int iters, drot_qr_tile *best, double *best_us_per_iter);
+
+/* The cache as text, one "n_rows n_cols vec lanes_per_col col_steps
+ * store_policy unroll us_per_iter" line per shape (load merges). */
+drot_status drot_qr_tile_cache_set(int n_rows, int n_cols, const drot_qr_tile *tile);
+drot_status drot_qr_tile_cache_save(const char *path);
+drot_status drot_qr_tile_cache_load(const char *path);
+void drot_qr_tile_cache_clear(void);
+
 /* ------------------------------------------------------- cost matrices */
 
 /* Squared-Euclidean cost between point clouds, written in the solver layout:

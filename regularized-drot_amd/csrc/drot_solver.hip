@@ -19,7 +19,11 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <fstream>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -102,22 +106,13 @@ void layout(Plan &P) {
     P.total = off;
 }
 
-// QR tile choice (measured on MI355X at 3000^2 and 10000^2): 16-byte accesses,
-// 32 lanes per column for tall matrices, write-through X stores, two batches
-// of two column steps in flight, about 4096 elements per block, and at least
-// ~1024 blocks when the matrix allows (256 CUs, several blocks resident each).
-// The strided-axis partials cost 8 bytes per row per column tile.
-Plan plan_qr(int n_rows, int n_cols, bool vec4_ok) {
+// Plan of a QR solve with a given pass tile; nstep <= 0 picks the column steps.
+Plan plan_qr_tile(int n_rows, int n_cols, QrTile t, int nstep) {
     Plan P;
     P.kind = kQR;
     P.n_rows = n_rows;
     P.n_cols = n_cols;
-    const bool v4 = vec4_ok && (n_rows % 4 == 0);
-    int nstep = 0;
-    if (!qr_tile_from_env(v4, &P.qt, &nstep)) {
-        if (v4) P.qt = QrTile{4, n_rows >= 1024 ? 32 : 16, 2, 2};
-        else P.qt = QrTile{1, n_rows >= 256 ? 64 : (n_rows >= 32 ? 32 : 16), 2, 2};
-    }
+    P.qt = t;
     const int tr = P.qt.lpc * P.qt.vec;
     const int cpw = kWave / P.qt.lpc;
     const int tc_min = kWavesPerBlock * cpw;
@@ -137,6 +132,62 @@ Plan plan_qr(int n_rows, int n_cols, bool vec4_ok) {
     P.cb = ceil_div(n_cols, kFinRows);
     layout(P);
     return P;
+}
+
+// Tiles chosen by the autotuner, per shape, for the whole process.
+struct TunedTile {
+    QrTile t;
+    int nstep;
+    double us;
+};
+std::mutex g_tune_mu;
+std::map<std::pair<int, int>, TunedTile> g_tuned;
+
+bool tuned_tile(int n_rows, int n_cols, bool v4, QrTile *t, int *nstep) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find({n_rows, n_cols});
+    if (it == g_tuned.end()) return false;
+    if (it->second.t.vec == 4 && !v4) return false;  // unaligned buffers: heuristic scalar tile
+    *t = it->second.t;
+    *nstep = it->second.nstep;
+    return true;
+}
+
+// QR tile choice (measured on MI355X at 3000^2 and 10000^2): 16-byte accesses,
+// 32 lanes per column for tall matrices, write-through X stores, two batches
+// of two column steps in flight, about 4096 elements per block, and at least
+// ~1024 blocks when the matrix allows (256 CUs, several blocks resident each).
+// The strided-axis partials cost 8 bytes per row per column tile.  A tile set
+// by the autotuner for this shape (or DROT_QR_TILE) takes precedence.
+Plan plan_qr(int n_rows, int n_cols, bool vec4_ok) {
+    const bool v4 = vec4_ok && (n_rows % 4 == 0);
+    QrTile t;
+    int nstep = 0;
+    if (!qr_tile_from_env(v4, &t, &nstep) && !tuned_tile(n_rows, n_cols, v4, &t, &nstep)) {
+        if (v4) t = QrTile{4, n_rows >= 1024 ? 32 : 16, 2, 2};
+        else t = QrTile{1, n_rows >= 256 ? 64 : (n_rows >= 32 ? 32 : 16), 2, 2};
+        nstep = 0;
+    }
+    return plan_qr_tile(n_rows, n_cols, t, nstep);
+}
+
+// Candidate tiles of the autotuner for one shape.
+std::vector<std::pair<QrTile, int>> tune_candidates(int n_rows, int n_cols, bool v4) {
+    std::vector<std::pair<QrTile, int>> out;
+    for (int vec : {4, 1}) {
+        if (vec == 4 && !v4) continue;
+        for (int lpc : {16, 32, 64}) {
+            if (lpc * vec > 4 * std::max(n_rows, 16)) continue;  // mostly padding
+            for (int nt : {2, 0})
+                for (int u : {2, 4})
+                    for (int nstep : {1, 2, 4}) {
+                        const int tc = nstep * kWavesPerBlock * (kWave / lpc);
+                        if (tc > 4 * std::max(n_cols, 4)) continue;
+                        out.push_back({QrTile{vec, lpc, nt, u}, nstep});
+                    }
+        }
+    }
+    return out;
 }
 
 constexpr size_t kGlLdsLimit = 64 * 1024 - 256;  // leaves room for static LDS
@@ -1548,6 +1599,147 @@ drot_status drot_sqdist_backward_f32(int n_problems, const drot_cost_problem *pr
     A.argmax = const_cast<int *>(argmax);
     A.normalize = normalize ? 1 : 0;
     DROT_HIP_CHECK(launch_cost_backward(A, s));
+    return DROT_OK;
+}
+
+drot_status drot_qr_tile_for(int n_rows, int n_cols, int vec4_ok, drot_qr_tile *out) {
+    if (!out || n_rows <= 0 || n_cols <= 0)
+        return fail(DROT_ERR_INVALID_ARGUMENT, "bad arguments to drot_qr_tile_for");
+    const Plan P = plan_qr(n_rows, n_cols, vec4_ok != 0);
+    out->vec = P.qt.vec;
+    out->lanes_per_col = P.qt.lpc;
+    out->col_steps = P.nstep;
+    out->store_policy = P.qt.nt;
+    out->unroll = P.qt.u;
+    return DROT_OK;
+}
+
+drot_status drot_qr_tile_cache_set(int n_rows, int n_cols, const drot_qr_tile *tile) {
+    if (!tile || n_rows <= 0 || n_cols <= 0)
+        return fail(DROT_ERR_INVALID_ARGUMENT, "bad arguments to drot_qr_tile_cache_set");
+    const QrTile t{tile->vec, tile->lanes_per_col, tile->store_policy, tile->unroll};
+    if (!qr_tile_supported(t) || tile->col_steps < 1 || tile->col_steps > 64)
+        return fail(DROT_ERR_INVALID_ARGUMENT, "unsupported tile");
+    if (t.vec == 4 && n_rows % 4 != 0)
+        return fail(DROT_ERR_INVALID_ARGUMENT, "vec 4 needs n_rows % 4 == 0");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[{n_rows, n_cols}] = TunedTile{t, tile->col_steps, 0.0};
+    return DROT_OK;
+}
+
+void drot_qr_tile_cache_clear(void) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned.clear();
+}
+
+drot_status drot_qr_tile_cache_save(const char *path) {
+    if (!path) return fail(DROT_ERR_INVALID_ARGUMENT, "path is NULL");
+    std::ofstream f(path);
+    if (!f) return fail(DROT_ERR_INVALID_ARGUMENT, std::string("cannot write ") + path);
+    f << "# n_rows n_cols vec lanes_per_col col_steps store_policy unroll us_per_iter\n";
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (const auto &kv : g_tuned)
+        f << kv.first.first << ' ' << kv.first.second << ' ' << kv.second.t.vec << ' '
+          << kv.second.t.lpc << ' ' << kv.second.nstep << ' ' << kv.second.t.nt << ' '
+          << kv.second.t.u << ' ' << kv.second.us << '\n';
+    return f ? DROT_OK : fail(DROT_ERR_INVALID_ARGUMENT, std::string("write failed: ") + path);
+}
+
+drot_status drot_qr_tile_cache_load(const char *path) {
+    if (!path) return fail(DROT_ERR_INVALID_ARGUMENT, "path is NULL");
+    std::ifstream f(path);
+    if (!f) return fail(DROT_ERR_INVALID_ARGUMENT, std::string("cannot read ") + path);
+    std::string line;
+    std::map<std::pair<int, int>, TunedTile> got;
+    int lineno = 0;
+    while (std::getline(f, line)) {
+        lineno++;
+        if (line.empty() || line[0] == '#') continue;
+        int nr, nc, v, l, ns, nt, u;
+        double us = 0.0;
+        if (sscanf(line.c_str(), "%d %d %d %d %d %d %d %lf", &nr, &nc, &v, &l, &ns, &nt, &u, &us) < 7)
+            return fail(DROT_ERR_INVALID_ARGUMENT, std::string(path) + ":" + std::to_string(lineno) +
+                                                       ": malformed line");
+        const QrTile t{v, l, nt, u};
+        if (nr <= 0 || nc <= 0 || !qr_tile_supported(t) || ns < 1 || ns > 64 ||
+            (v == 4 && nr % 4 != 0))
+            return fail(DROT_ERR_INVALID_ARGUMENT, std::string(path) + ":" + std::to_string(lineno) +
+                                                       ": unsupported tile");
+        got[{nr, nc}] = TunedTile{t, ns, us};
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (const auto &kv : got) g_tuned[kv.first] = kv.second;
+    return DROT_OK;
+}
+
+drot_status drot_qr_autotune_workspace_size(int n_rows, int n_cols, size_t *bytes) {
+    if (!bytes || n_rows <= 0 || n_cols <= 0)
+        return fail(DROT_ERR_INVALID_ARGUMENT, "bad arguments to drot_qr_autotune_workspace_size");
+    size_t mx = 0;
+    for (const auto &cand : tune_candidates(n_rows, n_cols, n_rows % 4 == 0))
+        mx = std::max(mx, plan_qr_tile(n_rows, n_cols, cand.first, cand.second).total);
+    size_t ws = 0;
+    drot_qr_workspace_size(n_rows, n_cols, &ws);
+    *bytes = std::max(mx, ws);
+    return DROT_OK;
+}
+
+drot_status drot_qr_autotune_f32(const float *c, const float *p, const float *q, int n_rows,
+                                 int n_cols, const drot_qr_params *prm, float *x,
+                                 void *workspace, size_t workspace_bytes, void *stream,
+                                 int iters, drot_qr_tile *best, double *best_us) {
+    drot_status s = check_common(c, p, q, n_rows, n_cols, x, 0);
+    if (s != DROT_OK) return s;
+    if (!prm || !workspace) return fail(DROT_ERR_INVALID_ARGUMENT, "null pointer argument");
+    if (!(prm->scale > 0.0f)) return fail(DROT_ERR_INVALID_ARGUMENT, "scale must be > 0");
+    // iterations k_lo..k_hi with k_hi sized to a few milliseconds of work, so
+    // the difference of the two run times measures the iterations alone
+    const double est_us = 6.0 + (double)n_rows * (double)n_cols * 10.0 / 5.0e6;
+    const int k_hi = std::max(std::max(8, iters), std::min(4000, (int)(4000.0 / est_us))) & ~1;
+    const int k_lo = std::max(2, (k_hi / 5) & ~1);
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const bool v4 = aligned16(c) && aligned16(x) && n_rows % 4 == 0;
+    double best_t = 1e300;
+    std::pair<QrTile, int> arg{};
+    bool any = false;
+    auto timed = [&](const Plan &P, int k, double *sec) -> drot_status {
+        SolveIn in{c, p, q, x, prm->step_size, prm->scale, 0.0f, k, 0.0f, prm->warm_start};
+        const auto t0 = std::chrono::steady_clock::now();
+        const drot_status r = run(P, in, workspace, st, nullptr);
+        *sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return r;
+    };
+    for (const auto &cand : tune_candidates(n_rows, n_cols, v4)) {
+        const Plan P = plan_qr_tile(n_rows, n_cols, cand.first, cand.second);
+        if (P.total > workspace_bytes) continue;
+        double per = 1e300;
+        for (int rep = 0; rep < 3; rep++) {
+            double t_lo = 0.0, t_hi = 0.0;
+            if ((s = timed(P, k_lo, &t_lo)) != DROT_OK) return s;
+            if ((s = timed(P, k_hi, &t_hi)) != DROT_OK) return s;
+            const double d = (t_hi - t_lo) / (double)(k_hi - k_lo);
+            if (d > 0.0) per = std::min(per, d);
+        }
+        if (per < best_t) {
+            best_t = per;
+            arg = cand;
+            any = true;
+        }
+    }
+    if (!any) return fail(DROT_ERR_WORKSPACE_TOO_SMALL, "no candidate tile fits the workspace");
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[{n_rows, n_cols}] = TunedTile{arg.first, arg.second, best_t * 1e6};
+    }
+    if (best) {
+        const Plan P = plan_qr_tile(n_rows, n_cols, arg.first, arg.second);
+        best->vec = P.qt.vec;
+        best->lanes_per_col = P.qt.lpc;
+        best->col_steps = P.nstep;
+        best->store_policy = P.qt.nt;
+        best->unroll = P.qt.u;
+    }
+    if (best_us) *best_us = best_t * 1e6;
     return DROT_OK;
 }
 

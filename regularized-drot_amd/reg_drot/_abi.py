@@ -30,6 +30,9 @@ EXPORTS = (
     "drot_qr_backward_workspace_size", "drot_qr_backward_f32",
     "drot_qr_batch_backward_workspace_size", "drot_qr_batch_backward_f32",
     "drot_sqdist_workspace_size", "drot_sqdist_f32", "drot_sqdist_backward_f32",
+    "drot_qr_tile_for", "drot_qr_autotune_workspace_size", "drot_qr_autotune_f32",
+    "drot_qr_tile_cache_set", "drot_qr_tile_cache_save", "drot_qr_tile_cache_load",
+    "drot_qr_tile_cache_clear",
 )
 
 DROT_OK = 0
@@ -84,6 +87,15 @@ class CostProblem(ctypes.Structure):
     _fields_ = [("n_rows", ctypes.c_int), ("n_cols", ctypes.c_int), ("ld", ctypes.c_int),
                 ("mat_offset", ctypes.c_longlong), ("src_offset", ctypes.c_longlong),
                 ("tgt_offset", ctypes.c_longlong)]
+
+
+class QrTile(ctypes.Structure):
+    _fields_ = [("vec", ctypes.c_int), ("lanes_per_col", ctypes.c_int),
+                ("col_steps", ctypes.c_int), ("store_policy", ctypes.c_int),
+                ("unroll", ctypes.c_int)]
+
+    def as_tuple(self):
+        return (self.vec, self.lanes_per_col, self.col_steps, self.store_policy, self.unroll)
 
 
 class DrotError(RuntimeError):
@@ -154,7 +166,19 @@ def load(path: str = LIB_PATH):
     lib.drot_sqdist_f32.argtypes = [c_int, CP, c_int, vp, vp, c_int, vp, vp, vp, vp, c_size, vp]
     lib.drot_sqdist_backward_f32.argtypes = [c_int, CP, c_int, vp, vp, vp, c_int, vp, vp, vp, vp,
                                              vp, c_size, vp]
-    for name in ("drot_sqdist_workspace_size", "drot_sqdist_f32", "drot_sqdist_backward_f32",
+    TP = P(QrTile)
+    lib.drot_qr_tile_for.argtypes = [c_int, c_int, c_int, TP]
+    lib.drot_qr_autotune_workspace_size.argtypes = [c_int, c_int, P(c_size)]
+    lib.drot_qr_autotune_f32.argtypes = [vp, vp, vp, c_int, c_int, P(QrParams), vp, vp, c_size,
+                                         vp, c_int, TP, P(ctypes.c_double)]
+    lib.drot_qr_tile_cache_set.argtypes = [c_int, c_int, TP]
+    lib.drot_qr_tile_cache_save.argtypes = [ctypes.c_char_p]
+    lib.drot_qr_tile_cache_load.argtypes = [ctypes.c_char_p]
+    lib.drot_qr_tile_cache_clear.argtypes = []
+    lib.drot_qr_tile_cache_clear.restype = None
+    for name in ("drot_qr_tile_for", "drot_qr_autotune_workspace_size", "drot_qr_autotune_f32",
+                 "drot_qr_tile_cache_set", "drot_qr_tile_cache_save", "drot_qr_tile_cache_load",
+                 "drot_sqdist_workspace_size", "drot_sqdist_f32", "drot_sqdist_backward_f32",
                  "drot_qr_backward_workspace_size", "drot_qr_backward_f32",
                  "drot_qr_batch_backward_workspace_size", "drot_qr_batch_backward_f32",
                  "drot_qr_params_from_rho", "drot_qr_params_from_step",
@@ -354,3 +378,44 @@ def sqdist_backward(probs, dim, src, tgt, grad_c, normalize, scale, argmax, grad
                                           int(bool(normalize)), scale or None, argmax or None,
                                           grad_src, grad_tgt, ws, ws_bytes, stream),
           "drot_sqdist_backward_f32")
+
+
+def qr_tile_for(n_rows, n_cols, vec4_ok=True) -> QrTile:
+    t = QrTile()
+    check(load().drot_qr_tile_for(n_rows, n_cols, int(bool(vec4_ok)), ctypes.byref(t)),
+          "drot_qr_tile_for")
+    return t
+
+
+def qr_tile_cache_set(n_rows, n_cols, tile: QrTile):
+    check(load().drot_qr_tile_cache_set(n_rows, n_cols, ctypes.byref(tile)),
+          "drot_qr_tile_cache_set")
+
+
+def qr_tile_cache_save(path):
+    check(load().drot_qr_tile_cache_save(os.fsencode(path)), "drot_qr_tile_cache_save")
+
+
+def qr_tile_cache_load(path):
+    check(load().drot_qr_tile_cache_load(os.fsencode(path)), "drot_qr_tile_cache_load")
+
+
+def qr_tile_cache_clear():
+    load().drot_qr_tile_cache_clear()
+
+
+def qr_autotune_workspace_size(n_rows, n_cols) -> int:
+    b = ctypes.c_size_t()
+    check(load().drot_qr_autotune_workspace_size(n_rows, n_cols, ctypes.byref(b)),
+          "drot_qr_autotune_workspace_size")
+    return b.value
+
+
+def qr_autotune(c, p, q, n_rows, n_cols, prm: QrParams, x, ws, ws_bytes, stream, iters=0):
+    """Device addresses; returns (QrTile, us_per_iter)."""
+    t = QrTile()
+    us = ctypes.c_double()
+    check(load().drot_qr_autotune_f32(c, p, q, n_rows, n_cols, ctypes.byref(prm), x, ws, ws_bytes,
+                                      stream, int(iters), ctypes.byref(t), ctypes.byref(us)),
+          "drot_qr_autotune_f32")
+    return t, us.value

@@ -104,3 +104,36 @@ def test_argument_errors_are_status_codes():
                              None, ctypes.byref(r))
     assert st == 2  # workspace too small
     assert L.drot_status_string(3) == b"unsupported shape"
+
+
+def test_tile_cache_roundtrip(tmp_path):
+    from reg_drot import _abi
+    _abi.qr_tile_cache_clear()
+    default = _abi.qr_tile_for(3000, 3000).as_tuple()
+    t = _abi.QrTile(4, 16, 3, 0, 4)
+    _abi.qr_tile_cache_set(3000, 3000, t)
+    assert _abi.qr_tile_for(3000, 3000).as_tuple() == (4, 16, 3, 0, 4)
+    # unaligned buffers cannot take a 16-byte tile: the built-in scalar choice is used
+    assert _abi.qr_tile_for(3000, 3000, vec4_ok=False).vec == 1
+    path = tmp_path / "tiles.txt"
+    _abi.qr_tile_cache_save(str(path))
+    _abi.qr_tile_cache_clear()
+    assert _abi.qr_tile_for(3000, 3000).as_tuple() == default
+    _abi.qr_tile_cache_load(str(path))
+    assert _abi.qr_tile_for(3000, 3000).as_tuple() == (4, 16, 3, 0, 4)
+    # the tuned tile changes the workspace the shape needs
+    assert _abi.qr_workspace_size(3000, 3000) > 0
+    assert _abi.qr_autotune_workspace_size(3000, 3000) >= _abi.qr_workspace_size(3000, 3000)
+    _abi.qr_tile_cache_clear()
+
+
+def test_tile_cache_rejects_bad_tiles(tmp_path):
+    from reg_drot import _abi
+    with pytest.raises(_abi.DrotError, match="vec 4"):
+        _abi.qr_tile_cache_set(1001, 10, _abi.QrTile(4, 16, 1, 2, 2))
+    with pytest.raises(_abi.DrotError, match="unsupported"):
+        _abi.qr_tile_cache_set(1000, 10, _abi.QrTile(2, 16, 1, 2, 2))
+    bad = tmp_path / "bad.txt"
+    bad.write_text("1000 10 4 16\n")
+    with pytest.raises(_abi.DrotError, match="malformed"):
+        _abi.qr_tile_cache_load(str(bad))
