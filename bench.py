@@ -9,8 +9,11 @@ value = algorithmic HBM throughput of the whole solve in GB/s:
     bytes = sum over executed iterations of 12*m*n (even k: read C and X,
             write X) or 8*m*n (odd k: read X, write X -- the even/odd -sC
             folding of core/kernel_qr.hpp is kept), divided by wall time.
-The roofline block times the fused pass kernel itself with HIP events on the
-solver's stream (library timing mode) over the timed steps.  cpu_baseline is
+The roofline block times the fused pass kernel itself in the library's timing
+mode over the timed steps: on sampled launches every block of the pass stores
+its start and end on the device's constant-rate clock (the launch's span is the
+earliest start to the latest end), and HIP events bracket every launch on the
+solver's stream.  cpu_baseline is
 the CPU oracle (a C port of the same iteration, OpenMP over all host threads)
 timed on a bounded sample of iterations of the same workload.
 
@@ -257,18 +260,24 @@ def main():
 
     # pass-kernel roofline: same steps again in timing mode (events on the stream)
     _abi.set_timing(True)
-    even_ms = odd_ms = 0.0
-    even_n = odd_n = 0
+    even_ms = odd_ms = even_ev = odd_ev = 0.0
+    even_n = odd_n = even_en = odd_en = 0
     for _ in range(max(1, args.steps)):
         r = run_once()
         even_ms += r.pass_ms[0]
         odd_ms += r.pass_ms[1]
         even_n += r.pass_count[0]
         odd_n += r.pass_count[1]
+        even_ev += r.pass_ms_events[0]
+        odd_ev += r.pass_ms_events[1]
+        even_en += r.pass_count_events[0]
+        odd_en += r.pass_count_events[1]
     _abi.set_timing(False)
     n = N_ROWS * N_COLS
     even_avg_us = even_ms / max(even_n, 1) * 1e3
     odd_avg_us = odd_ms / max(odd_n, 1) * 1e3
+    even_ev_us = even_ev / max(even_en, 1) * 1e3
+    odd_ev_us = odd_ev / max(odd_en, 1) * 1e3
     achieved = 12.0 * n / (even_avg_us * 1e-6) / 1e9  # even pass: the dominant kernel
     odd_gbs = 8.0 * n / (odd_avg_us * 1e-6) / 1e9
 
@@ -309,6 +318,13 @@ def main():
                       % (tile[0], tile[1], tile[3], tile[4], tile[2]),
             "avg_us_even": round(even_avg_us, 3),
             "avg_us_odd": round(odd_avg_us, 3),
+            "timing": "kernel execution span per launch on the device clock (earliest block "
+                      "start to latest block end) averaged over the sampled working launches "
+                      "(k %% 8 < 2) of the timed solves (%d even); HIP events bracketing "
+                      "every launch on the solver's stream give avg_us_even_events (they "
+                      "include the dispatch)" % even_n,
+            "avg_us_even_events": round(even_ev_us, 3),
+            "avg_us_odd_events": round(odd_ev_us, 3),
             "odd_pass_GBs": round(odd_gbs, 1),
         },
     }

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define DROT_ABI_VERSION 1
+#define DROT_ABI_VERSION 2
 
 typedef enum drot_status {
     DROT_OK = 0,
@@ -72,10 +72,17 @@ typedef struct drot_result {
                               the last iteration that ran the stopping test    */
     float run_ms;          /* wrappers only: solve time (as dur_in_ms)         */
     float prep_ms;         /* wrappers only: alloc + copies + init             */
-    double pass_ms[2];     /* timing mode only (drot_set_timing): summed device
-                              time of the fused pass kernels that did work,
-                              [0] = QR even iterations / GL, [1] = QR odd     */
-    int pass_count[2];     /* number of passes summed in pass_ms               */
+    double pass_ms[2];     /* timing mode only (drot_set_timing): summed
+                              execution time of sampled fused-pass launches that
+                              did work ([0] = QR even iterations / GL, [1] = QR
+                              odd): per launch, earliest block start to latest
+                              block end on the device's constant-rate clock;
+                              launches k with k % 8 < 2 are sampled             */
+    int pass_count[2];     /* number of launches summed in pass_ms             */
+    double pass_ms_events[2]; /* every working pass launch timed by HIP events
+                              bracketing it on the solve's stream (these
+                              include the dispatch of the launch)              */
+    int pass_count_events[2];
 } drot_result;
 
 /* ---------------------------------------------------------------- params */

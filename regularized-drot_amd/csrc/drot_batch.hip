@@ -27,6 +27,7 @@ __device__ inline IterCtl batch_ctl(const QrBatchArgs &A, const BatchProblem &P,
     c.obj_even_only = 1;
     c.n_rows = P.n_rows;
     c.n_cols = P.n_cols;
+    c.tspan = nullptr;
     return c;
 }
 

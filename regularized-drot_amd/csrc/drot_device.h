@@ -59,7 +59,20 @@ struct IterCtl {
     int stop_rule;       // StopRule
     int obj_even_only;   // QR: objective only on even iterations
     int n_rows, n_cols;
+    // timing mode, sampled launches only (else null): block b stores its start
+    // and end on the constant-rate device clock at tspan[2b], tspan[2b+1]
+    unsigned long long *tspan;
 };
+
+// Records this block's execution span (plain stores, one slot pair per block).
+__device__ __forceinline__ void record_span(unsigned long long *tspan, unsigned long long t0) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t1 = (unsigned long long)wall_clock64();
+        tspan[2 * blockIdx.x] = t0;
+        tspan[2 * blockIdx.x + 1] = t1;
+    }
+}
 
 // ---------------------------------------------------------------- reductions
 

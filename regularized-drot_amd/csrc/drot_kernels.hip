@@ -113,6 +113,7 @@ template <int VEC, int RCH>
 __global__ __launch_bounds__(kBlock) void gl_pass_kernel(GlPassArgs A) {
     extern __shared__ double s_dyn[];
     __shared__ double s_red[3 * kWavesPerBlock];
+    const unsigned long long t0 = A.ctl.tspan ? (unsigned long long)wall_clock64() : 0ull;
     if (A.ctl.st->done) return;
     const IterStart is = iteration_prologue(A.ctl, s_red, blockIdx.x == 0);
     if (is.finish) return;
@@ -232,6 +233,7 @@ __global__ __launch_bounds__(kBlock) void gl_pass_kernel(GlPassArgs A) {
     }
     const double o = block_sum(oacc, s_red);
     if (threadIdx.x == 0) A.pobj[blockIdx.x] = o;
+    if (A.ctl.tspan) record_span(A.ctl.tspan, t0);
 }
 
 // ----------------------------------------------------------------- finalize

@@ -20,6 +20,7 @@ __global__ __launch_bounds__(kBlock) void qr_pass_kernel(QrPassArgs A) {
     __shared__ double s_row[kWavesPerBlock * CPW][TR];
     __shared__ double s_red[3 * kWavesPerBlock];
 
+    const unsigned long long t0 = A.ctl.tspan ? (unsigned long long)wall_clock64() : 0ull;
     if (A.ctl.st->done) return;
     // XCD-aware tile order: blocks are dispatched round-robin over the 8 XCDs
     // (block b on XCD b % 8), so give each XCD a contiguous run of tiles in
@@ -50,6 +51,7 @@ __global__ __launch_bounds__(kBlock) void qr_pass_kernel(QrPassArgs A) {
     J.step = A.step;
     J.kappa = A.kappa;
     qr_tile<VEC, LPC, EVEN, ST, U>(J, A.ctl, blockIdx.x == 0, s_row, s_red);
+    if (A.ctl.tspan) record_span(A.ctl.tspan, t0);
 }
 
 // ----------------------------------------------------------------- launch

@@ -457,11 +457,38 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
     // (at most two) chunks in flight, and a half is read back before reuse.
     const bool timing = g_timing;
     std::vector<float> pass_ms;
+    // Device-clock spans (timing mode): launches with k % 8 < 2 (one even, one
+    // odd of every eight) are sampled, each into its own [2 x grid] slot block,
+    // at most kMaxSpanBytes in total; the other launches run uninstrumented.
+    unsigned long long *tspan = nullptr;
+    struct SpanFree {
+        unsigned long long *p;
+        ~SpanFree() {
+            if (p) (void)hipFree(p);
+        }
+    } span_guard{nullptr};
+    std::vector<int> span_slot;  // per launch: sample index or -1
+    int n_samples = 0;
     if (timing) {
         drot_status ts = timing_pool(2 * chunk);
         if (ts != DROT_OK) return ts;
         pass_ms.assign((size_t)std::max(n_launch, 1), 0.0f);
+        constexpr size_t kMaxSpanBytes = 64u << 20;
+        const size_t per = sizeof(unsigned long long) * 2 * (size_t)P.grid_pass;
+        const int cap = (int)std::max<size_t>(2, kMaxSpanBytes / per);
+        span_slot.assign((size_t)std::max(n_launch, 1), -1);
+        for (int k = 0; k < n_launch && n_samples < cap; k++)
+            if (k % 8 < 2) span_slot[(size_t)k] = n_samples++;
+        if (n_samples > 0) {
+            DROT_HIP_CHECK(hipMalloc((void **)&tspan, per * n_samples));
+            span_guard.p = tspan;
+            DROT_HIP_CHECK(hipMemsetAsync(tspan, 0, per * n_samples, stream));
+        }
     }
+    auto span_ptr = [&](int k) -> unsigned long long * {
+        if (!tspan || k >= (int)span_slot.size() || span_slot[(size_t)k] < 0) return nullptr;
+        return tspan + (size_t)span_slot[(size_t)k] * 2 * (size_t)P.grid_pass;
+    };
     int chunk_start[2] = {0, 0}, chunk_n[2] = {0, 0};
     auto harvest = [&](int half) -> drot_status {
         for (int t = 0; t < chunk_n[half]; t++) {
@@ -491,9 +518,11 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
             }
             if (P.kind == kQR) {
                 qa.ctl.k = k;
+                qa.ctl.tspan = span_ptr(k);
                 DROT_HIP_CHECK(launch_qr_pass(P.qt, (k % 2) == 0, P.grid_pass, qa, stream, e0, e1));
             } else {
                 ga.ctl.k = k;
+                ga.ctl.tspan = span_ptr(k);
                 DROT_HIP_CHECK(launch_gl_pass(P.gt, P.grid_pass, P.lds, ga, stream, e0, e1));
             }
             if (k < in.max_iters) {
@@ -526,10 +555,32 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
     DROT_HIP_CHECK(hipMemcpyAsync(hs, d.st, sizeof(SolverState), hipMemcpyDeviceToHost,
                                   stream));
     DROT_HIP_CHECK(hipStreamSynchronize(stream));
+    std::vector<double> span_us;  // per launch, -1 when not sampled
     if (timing) {
         for (int h = 0; h < 2; h++) {
             drot_status st = harvest(h);
             if (st != DROT_OK) return st;
+        }
+        span_us.assign((size_t)std::max(n_launch, 1), -1.0);
+        if (n_samples > 0) {
+            const size_t per = 2 * (size_t)P.grid_pass;
+            std::vector<unsigned long long> hsp(per * n_samples);
+            DROT_HIP_CHECK(hipMemcpy(hsp.data(), tspan, sizeof(unsigned long long) * hsp.size(),
+                                     hipMemcpyDeviceToHost));
+            int dev = 0, khz = 0;
+            DROT_HIP_CHECK(hipGetDevice(&dev));
+            DROT_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+            for (size_t k = 0; k < span_us.size(); k++) {
+                if (span_slot[k] < 0) continue;
+                const unsigned long long *b = hsp.data() + per * (size_t)span_slot[k];
+                unsigned long long t0 = ~0ull, t1 = 0;
+                for (size_t e = 0; e < per; e += 2) {
+                    if (b[e] == 0) continue;  // block did no work
+                    t0 = std::min(t0, b[e]);
+                    t1 = std::max(t1, b[e + 1]);
+                }
+                if (t1 > t0 && khz > 0) span_us[k] = (double)(t1 - t0) * 1000.0 / (double)khz;
+            }
         }
     }
     if (res) {
@@ -542,12 +593,18 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
         res->run_ms = 0.0f;
         res->prep_ms = 0.0f;
         res->pass_ms[0] = res->pass_ms[1] = 0.0;
+        res->pass_ms_events[0] = res->pass_ms_events[1] = 0.0;
         res->pass_count[0] = res->pass_count[1] = 0;
+        res->pass_count_events[0] = res->pass_count_events[1] = 0;
         if (timing) {
             for (int k = 0; k < hs->k_run && k < (int)pass_ms.size(); k++) {
                 const int par = (P.kind == kQR && (k % 2) == 1) ? 1 : 0;
-                res->pass_ms[par] += pass_ms[(size_t)k];
-                res->pass_count[par] += 1;
+                if (span_us[(size_t)k] >= 0.0) {
+                    res->pass_ms[par] += span_us[(size_t)k] * 1e-3;
+                    res->pass_count[par] += 1;
+                }
+                res->pass_ms_events[par] += pass_ms[(size_t)k];
+                res->pass_count_events[par] += 1;
             }
         }
     }

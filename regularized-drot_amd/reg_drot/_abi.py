@@ -36,6 +36,7 @@ EXPORTS = (
 )
 
 DROT_OK = 0
+ABI_VERSION = 2  # DROT_ABI_VERSION of the include/drot.h these prototypes follow
 
 
 class QrParams(ctypes.Structure):
@@ -61,12 +62,16 @@ class Result(ctypes.Structure):
                 ("iterations_run", ctypes.c_int), ("objective", ctypes.c_float),
                 ("objective_f64", ctypes.c_double), ("residual", ctypes.c_double),
                 ("run_ms", ctypes.c_float), ("prep_ms", ctypes.c_float),
-                ("pass_ms", ctypes.c_double * 2), ("pass_count", ctypes.c_int * 2)]
+                ("pass_ms", ctypes.c_double * 2), ("pass_count", ctypes.c_int * 2),
+                ("pass_ms_events", ctypes.c_double * 2),
+                ("pass_count_events", ctypes.c_int * 2)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
         d["pass_ms"] = list(self.pass_ms)
         d["pass_count"] = list(self.pass_count)
+        d["pass_ms_events"] = list(self.pass_ms_events)
+        d["pass_count_events"] = list(self.pass_count_events)
         return d
 
 
@@ -146,6 +151,9 @@ def load(path: str = LIB_PATH):
     lib.drot_last_error.restype = ctypes.c_char_p
     lib.drot_abi_version.argtypes = []
     lib.drot_abi_version.restype = c_int
+    if lib.drot_abi_version() != ABI_VERSION:
+        raise DrotError(f"{path} implements ABI {lib.drot_abi_version()}, expected {ABI_VERSION}; "
+                        "rebuild it")
     lib.drot_set_timing.argtypes = [c_int]
     lib.drot_set_timing.restype = None
     ll = ctypes.c_longlong

@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
     L = lib()
     for name in declared_functions():
         assert hasattr(L, name), name
-    assert L.drot_abi_version() == 1
+    assert L.drot_abi_version() == 2
 
 
 def test_no_cpu_fallback_symbols():
