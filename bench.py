@@ -119,6 +119,49 @@ def cpu_baseline(c, p, q, step, kappa, target_s):
             "us_per_iter": round(dt / (2 * pairs) * 1e6, 1)}
 
 
+def cpu_reference_case(dev, sample_s):
+    """The CPU-runnable reference case: QR 1000 x 1000, C iid U[0,1) (seed 0),
+    uniform marginals, r_weight 1e-3, rho 2, tol 1e-4, at most 10000 iterations.
+    GPU time to tolerance, and the numpy restatement of the reference's CPU
+    solver (npdrot) timed on a bounded sample of iterations."""
+    import numpy as np
+    import torch
+
+    import reg_drot
+    from oracle import npdrot
+
+    n = 1000
+    c_np = np.random.default_rng(0).random((n, n), dtype=np.float32)
+    p_np = np.full(n, 1.0 / n, np.float32)
+    c = torch.from_numpy(c_np).to(dev)
+    p = torch.from_numpy(p_np).to(dev)
+    reg_drot.quadratic_drot_ex(c, p, p, 2.0, 1e-3, 10000, 1e-4)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    _, info = reg_drot.quadratic_drot_ex(c, p, p, 2.0, 1e-3, 10000, 1e-4)
+    torch.cuda.synchronize(dev)
+    gpu_ms = (time.perf_counter() - t0) * 1e3
+    step = 2.0 / (2 * n)
+    reg = 1e-3 * (2 * n)
+    cf = np.asfortranarray(c_np.T, np.float64)
+    y0 = np.full((n, n), step)
+    t0 = time.perf_counter()
+    npdrot.regdrot(y0, cf, p_np, p_np, reg, max_iters=2, eps_abs=-1.0, step=step)
+    per = max((time.perf_counter() - t0) / 2, 1e-4)
+    iters = max(2, int(sample_s / per))
+    t0 = time.perf_counter()
+    npdrot.regdrot(y0, cf, p_np, p_np, reg, max_iters=iters, eps_abs=-1.0, step=step)
+    us = (time.perf_counter() - t0) / iters * 1e6
+    return {"workload": "QR-DROT 1000x1000 fp32, C iid U[0,1) seed 0, uniform marginals, "
+                        "r_weight 1e-3, rho 2, tol 1e-4, max 10000 iterations",
+            "gpu_time_to_tol_ms": round(gpu_ms, 3), "iterations": info["iterations_run"],
+            "converged": info["converged"],
+            "npdrot_numpy_us_per_iter": round(us, 1),
+            "npdrot_numpy_ms_for_same_iterations": round(us * info["iterations_run"] / 1e3, 1),
+            "npdrot_sample": f"{iters} iterations of the numpy restatement of npdrot.regdrot "
+                             "(fp64, 1 thread)"}
+
+
 def batched_fwd_bwd(dev, workload):
     """One batched and one looped forward+backward pass over a workload; returns
     times, solves/s and the largest gradient difference between the two."""
@@ -338,6 +381,7 @@ def main():
         s = np.float32(prm.step_size)
         k = np.float32(prm.scale)
         out["cpu_baseline"] = cpu_baseline(c_np, p_np, q_np, s, k, args.cpu_sample_s)
+        out["cpu_reference_case"] = cpu_reference_case(dev, min(10.0, args.cpu_sample_s))
     print(json.dumps(out))
 
 

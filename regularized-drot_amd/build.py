@@ -81,8 +81,22 @@ def build_ext() -> str:
     return EXT
 
 
+CLI = os.path.join(PKG_DIR, "drot_cli")
+
+
+def build_cli() -> str:
+    """The experiment command line (cli/drot_cli.cpp), a host program over libdrot.so."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE,
+           os.path.join(PKG_DIR, "cli", "drot_cli.cpp"), "-o", CLI + ".tmp",
+           "-L", PKG_DIR, "-Wl,-rpath,$ORIGIN", "-ldrot"]
+    _run(cmd)
+    os.replace(CLI + ".tmp", CLI)
+    return CLI
+
+
 def build_all(with_ext: bool = True) -> None:
     build_lib()
+    build_cli()
     if with_ext:
         build_ext()
 
