@@ -29,7 +29,13 @@ on the same GPU, for two workloads:
 The looped path runs cost, reg_drot.quadratic_drot_diff and backward problem by
 problem; both paths produce the same gradients (checked).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-batched]
+Further context blocks: cpu_reference_case (the CPU-runnable reference case,
+QR 1000 x 1000 with C ~ U[0,1): GPU time to tolerance and the numpy
+restatement of the reference's CPU solver), gl_reference_task (GL-DROT on the
+reference's published domain-adaptation task size) and host_wrapper_3000x3000
+(the host-pointer wrapper including both PCIe copies).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-batched] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -160,6 +166,84 @@ def cpu_reference_case(dev, sample_s):
             "npdrot_numpy_ms_for_same_iterations": round(us * info["iterations_run"] / 1e3, 1),
             "npdrot_sample": f"{iters} iterations of the numpy restatement of npdrot.regdrot "
                              "(fp64, 1 thread)"}
+
+
+def gl_reference_task(dev):
+    """GL-DROT on the domain-adaptation task of the reference's published GL
+    timings (1500 source points in 2 labelled Gaussian classes, 1000 target
+    points under a random affine map, C = sqdist / max, 2 groups, uniform
+    marginals, tol 1e-4), with the core lambda convention of the reference's
+    command-line program (core/drot_glr.hpp:385).  The reference reports
+    medians of 0.0619 s (r_weight 1e-3) and 0.0384 s (5e-3) over 50 datasets on
+    one V100 (README.md:70-71) with its own data generator: context only."""
+    import numpy as np
+    import torch
+
+    from reg_drot import _abi, problems
+
+    out = {}
+    n_src, n_tgt, groups = 1500, 1000, 2
+    for rw, published in ((1e-3, 0.0619), (5e-3, 0.0384)):
+        times, iters = [], []
+        for seed in range(5):
+            c_np, _ = problems.domain_adaptation(n_src, n_tgt, seed=seed)
+            c = torch.from_numpy(c_np).to(dev)
+            p = torch.full((n_src,), 1.0 / n_src, device=dev)
+            q = torch.full((n_tgt,), 1.0 / n_tgt, device=dev)
+            x = torch.empty_like(c)
+            step = 2.0 / (n_src + n_tgt)
+            prm = _abi.glr_params_from_step(step, rw, n_src, n_tgt, groups, 20000, 1e-4)
+            wsb = _abi.glr_workspace_size(n_src, n_tgt, groups)
+            ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+            stream = torch.cuda.current_stream(dev).cuda_stream
+            for rep in range(2):
+                torch.cuda.synchronize(dev)
+                t0 = time.perf_counter()
+                r = _abi.glr_solve(c.data_ptr(), p.data_ptr(), q.data_ptr(), n_src, n_tgt, prm,
+                                   x.data_ptr(), ws.data_ptr(), wsb, stream)
+                torch.cuda.synchronize(dev)
+                dt = time.perf_counter() - t0
+            times.append(dt)
+            iters.append(r.iterations_run)
+        out[f"r_weight_{rw:g}"] = {
+            "median_s": round(float(np.median(times)), 5),
+            "median_iterations": int(np.median(iters)),
+            "reference_v100_median_s": published}
+    out["workload"] = ("GL-DROT 1500x1000 fp32, 2 groups, domain-adaptation clouds (5 seeds), "
+                       "core lambda convention, tol 1e-4, solve time with inputs on the GPU")
+    return out
+
+
+def wrapper_round_trip(dev):
+    """The host-pointer wrapper (core/drot_qr.hpp:241-257 argument order) on the
+    3000 x 3000 workload: prep (allocation + host-to-device copies) and run
+    time, and the time including the copy of X back to the host."""
+    import ctypes
+
+    import numpy as np
+
+    from reg_drot import _abi, problems
+
+    lib = _abi.load()
+    c = np.ascontiguousarray(problems.cloud_cost(N_ROWS, N_COLS, dim=16, seed=0))
+    p = np.full(N_ROWS, 1.0 / N_ROWS, np.float32)
+    q = np.full(N_COLS, 1.0 / N_COLS, np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    dur, prep, obj = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+    it = ctypes.c_int()
+    step = RHO / (N_ROWS + N_COLS)
+    res = {}
+    for _ in range(2):
+        t0 = time.perf_counter()
+        xp = lib.drot_qr_wrapper_f32(c.ctypes.data_as(fp), p.ctypes.data_as(fp),
+                                     q.ctypes.data_as(fp), N_ROWS, N_COLS, step, R_WEIGHT,
+                                     MAX_ITERS, EPS, ctypes.byref(dur), ctypes.byref(prep),
+                                     ctypes.byref(it), ctypes.byref(obj), 1, 1)
+        wall = time.perf_counter() - t0
+        lib.drot_free(ctypes.cast(xp, ctypes.c_void_p))
+        res = {"prep_ms": round(prep.value, 3), "run_ms": round(dur.value, 3),
+               "wall_ms_with_copies": round(wall * 1e3, 3), "iterations": it.value}
+    return res
 
 
 def batched_fwd_bwd(dev, workload):
@@ -377,6 +461,8 @@ def main():
             "differentiable_64x512x512": batched_fwd_bwd(dev, "differentiable"),
             "mixed_256_m_n_64_2048": batched_fwd_bwd(dev, "mixed"),
         }
+    out["gl_reference_task"] = gl_reference_task(dev)
+    out["host_wrapper_3000x3000"] = wrapper_round_trip(dev)
     if not args.no_cpu:
         s = np.float32(prm.step_size)
         k = np.float32(prm.scale)

@@ -14,8 +14,9 @@
  *                  quadratic_regularizer_drot / group_lasso_regularizer_drot).
  *   *_wrapper_f32  host pointers, allocation, copies and timers inside; the
  *                  reference's core wrappers with the same argument order.
- * All functions are thread-safe for distinct workspaces.  Calls block until the
- * solve has finished on the given stream (as the reference's loops do).
+ * All functions are thread-safe for distinct workspaces.  Solves and backwards
+ * block until they have finished on the given stream (as the reference's loops
+ * do); the cost-matrix functions are asynchronous on their stream.
  */
 #ifndef DROT_H_
 #define DROT_H_
