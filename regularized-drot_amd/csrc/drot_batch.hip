@@ -126,9 +126,9 @@ __global__ __launch_bounds__(kBlock) void batch_finalize_kernel(QrBatchArgs A) {
     __shared__ double s_part[kBlock / kFinRows][kFinRows];
     __shared__ double s_red[3 * kWavesPerBlock];
     const BatchFin f = A.fins[blockIdx.x];
-    if (A.st[f.prob].done) return;
     const BatchProblem P = A.probs[f.prob];
     FinJob F;
+    F.done = &A.st[f.prob].done;
     F.prow = A.prow + P.prow_off;
     F.pcol = A.pcol + P.pcol_off;
     F.pobj = A.pobj + P.tile_off;

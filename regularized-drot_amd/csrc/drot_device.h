@@ -158,6 +158,7 @@ __device__ inline IterStart iteration_prologue(const IterCtl &C, double *s_red,
     bool conv = false;
     bool finish = k >= C.max_iters;
     if (k > 0) {
+        const float alpha_prev = st->alpha_slot[k & 1];  // after iteration k-2 (loaded early)
         double rr = 0.0, ss = 0.0, oo = 0.0;
         for (int t = threadIdx.x; t < C.n_fin; t += kBlock) {
             rr += C.pfin[3 * t];
@@ -168,7 +169,6 @@ __device__ inline IterStart iteration_prologue(const IterCtl &C, double *s_red,
         const double S = ss;
         const double obj = oo;
         const double res = sqrt(rr);
-        const float alpha_prev = st->alpha_slot[k & 1];  // after iteration k-2
         const float gamma =
             (float)((2.0 * (S - 1.0) - (double)alpha_prev) / (double)(C.n_rows + C.n_cols));
         const float alpha_new = (float)(((double)alpha_prev + 1.0) - S);

@@ -23,6 +23,7 @@ struct FinJob {
     const float *p, *q;
     float *a, *b, *ahat, *bhat;        // the problem's dual vectors
     double *pfin;                      // this block's 3 scalars
+    const int *done;                   // the problem's SolverState::done
     int n_rows, n_cols, n_rt, n_ct;
     int n_obj;                         // objective partials of the problem
     int rb;                            // row blocks of the problem
@@ -31,9 +32,13 @@ struct FinJob {
     int with_obj;
 };
 
+// The block is latency-bound: every global load (the done flag, the partials,
+// p/q, a/b and the objective slice) is issued before the first of them is
+// used, so the block pays one memory round trip before its stores.
 __device__ inline void finalize_block(const FinJob &A, double (*s_part)[kFinRows],
                                       double *s_red) {
     constexpr int kParts = kBlock / kFinRows;
+    constexpr int kBatch = 8;  // partial loads in flight per thread
     const int N = A.n_rows, M = A.n_cols;
     const int b = A.blk;
     const int part = threadIdx.x / kFinRows;
@@ -44,49 +49,55 @@ __device__ inline void finalize_block(const FinJob &A, double (*s_part)[kFinRows
     const int cnt = is_row ? A.n_ct : A.n_rt;
     const bool valid = idx < len;
     const double *src = (is_row ? A.prow : A.pcol) + (valid ? idx : 0);
-    double sum = 0.0;
-    if (valid) {
-        int t = part;
-        for (; t + (8 - 1) * kParts < cnt; t += 8 * kParts) {
-            double v[8];
+    const int done = *A.done;
+    double v[kBatch];
 #pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = src[(size_t)(t + u * kParts) * len];
-#pragma unroll
-            for (int u = 0; u < 8; u++) sum += v[u];
-        }
-        for (; t < cnt; t += kParts) sum += src[(size_t)t * len];
+    for (int u = 0; u < kBatch; u++) {
+        const int t = part + u * kParts;
+        v[u] = (valid && t < cnt) ? src[(size_t)t * len] : 0.0;
     }
-    s_part[part][li] = sum;
-    __syncthreads();
-    double res = 0.0, S = 0.0;
-    if (part == 0 && valid) {
-        double tot = 0.0;
-#pragma unroll
-        for (int q = 0; q < kParts; q++) tot += s_part[q][li];
-        if (is_row) {
-            const double d = tot - (double)A.p[idx];
-            const float rs = (float)d;
-            const float an = A.a[idx] - rs;
-            A.a[idx] = an;
-            A.ahat[idx] = an - rs;
-            res = d * d;
-        } else {
-            const double d = tot - (double)A.q[idx];
-            const float cs = (float)d;
-            const float bn = A.b[idx] - cs;
-            A.b[idx] = bn;
-            A.bhat[idx] = bn - cs;
-            res = d * d;
-            S = tot;
-        }
-    }
-    // this block's slice of the pass's objective partials
+    const bool owner = part == 0 && valid;
+    const float marg = owner ? (is_row ? A.p[idx] : A.q[idx]) : 0.0f;
+    const float acc = owner ? (is_row ? A.a[idx] : A.b[idx]) : 0.0f;
     double obj = 0.0;
-    if (A.with_obj) {
+    if (A.with_obj) {  // this block's slice of the pass's objective partials
         const int per = (A.n_obj + A.n_fin - 1) / A.n_fin;
         const int o0 = b * per;
         const int o1 = min(o0 + per, A.n_obj);
         for (int t = o0 + threadIdx.x; t < o1; t += kBlock) obj += A.pobj[t];
+    }
+    if (done) return;  // the solve ended: leave the dual state as it is
+    double sum = 0.0;
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) sum += v[u];
+    for (int t0 = part + kBatch * kParts; t0 < cnt; t0 += kBatch * kParts) {
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const int t = t0 + u * kParts;
+            v[u] = (valid && t < cnt) ? src[(size_t)t * len] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) sum += v[u];
+    }
+    s_part[part][li] = sum;
+    __syncthreads();
+    double res = 0.0, S = 0.0;
+    if (owner) {
+        double tot = 0.0;
+#pragma unroll
+        for (int q = 0; q < kParts; q++) tot += s_part[q][li];
+        const double d = tot - (double)marg;
+        const float r = (float)d;
+        const float an = acc - r;
+        if (is_row) {
+            A.a[idx] = an;
+            A.ahat[idx] = an - r;
+        } else {
+            A.b[idx] = an;
+            A.bhat[idx] = an - r;
+            S = tot;
+        }
+        res = d * d;
     }
     block_sum3(res, S, obj, s_red);
     if (threadIdx.x == 0) {

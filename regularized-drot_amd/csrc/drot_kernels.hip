@@ -241,8 +241,8 @@ __global__ __launch_bounds__(kBlock) void gl_pass_kernel(GlPassArgs A) {
 __global__ __launch_bounds__(kBlock) void finalize_kernel(FinalizeArgs A) {
     __shared__ double s_part[kBlock / kFinRows][kFinRows];
     __shared__ double s_red[3 * kWavesPerBlock];
-    if (A.st->done) return;
     FinJob F;
+    F.done = &A.st->done;
     F.prow = A.prow;
     F.pcol = A.pcol;
     F.pobj = A.pobj;
