@@ -36,6 +36,7 @@ reference's published domain-adaptation task size) and host_wrapper_3000x3000
 (the host-pointer wrapper including both PCIe copies).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-batched] [--no-cpu]
+                       [--no-context]
 """
 from __future__ import annotations
 
@@ -69,6 +70,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched forward+backward comparison")
+    ap.add_argument("--no-context", action="store_true",
+                    help="skip the GL task, host-wrapper and CPU reference-case blocks")
     ap.add_argument("--tune", action="store_true",
                     help="autotune the pass tile for the workload instead of loading the "
                          "measured tile table")
@@ -461,13 +464,15 @@ def main():
             "differentiable_64x512x512": batched_fwd_bwd(dev, "differentiable"),
             "mixed_256_m_n_64_2048": batched_fwd_bwd(dev, "mixed"),
         }
-    out["gl_reference_task"] = gl_reference_task(dev)
-    out["host_wrapper_3000x3000"] = wrapper_round_trip(dev)
+    if not args.no_context:
+        out["gl_reference_task"] = gl_reference_task(dev)
+        out["host_wrapper_3000x3000"] = wrapper_round_trip(dev)
     if not args.no_cpu:
         s = np.float32(prm.step_size)
         k = np.float32(prm.scale)
         out["cpu_baseline"] = cpu_baseline(c_np, p_np, q_np, s, k, args.cpu_sample_s)
-        out["cpu_reference_case"] = cpu_reference_case(dev, min(10.0, args.cpu_sample_s))
+        if not args.no_context:
+            out["cpu_reference_case"] = cpu_reference_case(dev, min(10.0, args.cpu_sample_s))
     print(json.dumps(out))
 
 

@@ -180,7 +180,9 @@ std::vector<std::pair<QrTile, int>> tune_candidates(int n_rows, int n_cols, bool
             if (lpc * vec > 4 * std::max(n_rows, 16)) continue;  // mostly padding
             for (int nt : {2, 0})
                 for (int u : {2, 4})
-                    for (int nstep : {1, 2, 4}) {
+                    for (int nstep : {1, 2, 3, 4, 5, 6, 7, 8, 12, 16}) {
+                        // a 4-step batch only pays with enough steps to fill it
+                        if (u == 4 && nstep < 4) continue;
                         const int tc = nstep * kWavesPerBlock * (kWave / lpc);
                         if (tc > 4 * std::max(n_cols, 4)) continue;
                         out.push_back({QrTile{vec, lpc, nt, u}, nstep});
