@@ -159,13 +159,33 @@ __device__ inline IterStart iteration_prologue(const IterCtl &C, double *s_red,
     bool finish = k >= C.max_iters;
     if (k > 0) {
         const float alpha_prev = st->alpha_slot[k & 1];  // after iteration k-2 (loaded early)
+        // every thread loads at most one finalize block's scalars; per-wave
+        // sums meet in LDS behind one barrier and every thread adds the wave
+        // values in the same fixed order (the scratch is next written after
+        // later barriers of the pass)
         double rr = 0.0, ss = 0.0, oo = 0.0;
         for (int t = threadIdx.x; t < C.n_fin; t += kBlock) {
             rr += C.pfin[3 * t];
             ss += C.pfin[3 * t + 1];
             oo += C.pfin[3 * t + 2];
         }
-        block_sum3(rr, ss, oo, s_red);
+        rr = wave_sum(rr);
+        ss = wave_sum(ss);
+        oo = wave_sum(oo);
+        const int wv = threadIdx.x / kWave;
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            s_red[wv] = rr;
+            s_red[kWavesPerBlock + wv] = ss;
+            s_red[2 * kWavesPerBlock + wv] = oo;
+        }
+        __syncthreads();
+        rr = ss = oo = 0.0;
+#pragma unroll
+        for (int q = 0; q < kWavesPerBlock; q++) {
+            rr += s_red[q];
+            ss += s_red[kWavesPerBlock + q];
+            oo += s_red[2 * kWavesPerBlock + q];
+        }
         const double S = ss;
         const double obj = oo;
         const double res = sqrt(rr);
