@@ -34,6 +34,7 @@ struct GlPassArgs {
     double *pcol;       // [n_cols]
     double *pobj;       // [n_ct]
     const int *bounds;  // [n_groups + 1] group row bounds
+    const int *row_group; // [n_rows] group of each row
     int n_rows, n_cols, n_groups;
     int n_ct, tc;       // column tiles, columns per tile
     int lane_per_group; // 1: one lane sums a group serially (small groups)
@@ -73,7 +74,8 @@ struct QrTile {
 
 struct GlTile {
     int vec;   // 1 or 4
-    int rch;   // row chunks per thread
+    int rch;   // row chunks per thread (per lane for the wave-per-column kernel)
+    int wave;  // 1: one wave per column (gl_wave_kernel), n_rows <= 2048
 };
 
 hipError_t launch_init(const InitArgs &a, hipStream_t s);

@@ -39,6 +39,17 @@ struct VecIO<1> {
     static __device__ __forceinline__ void store(float *p, const float (&v)[1]) { *p = v[0]; }
 };
 
+template <int VEC>
+__device__ __forceinline__ void load_ints(const int *p, int (&v)[VEC]) {
+    if constexpr (VEC == 4) {
+        const int4 t = *reinterpret_cast<const int4 *>(p);
+        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    } else {
+#pragma unroll
+        for (int e = 0; e < VEC; e++) v[e] = p[e];
+    }
+}
+
 // phi of one row / column: explicit array on the first iteration after an
 // initialisation, (hat + gamma) / n afterwards.  The two fp32 roundings are
 // those of `(ab_val - rs_val + gamma) / T(n)` in core/kernel_qr.hpp:431,448
@@ -99,6 +110,16 @@ __global__ __launch_bounds__(kBlock) void init_kernel(InitArgs A) {
 
 // ----------------------------------------------------------------- GL pass
 
+// Group row bounds are read from LDS when there are at most kGlBoundsLds
+// groups (copied at kernel start).
+constexpr int kGlBoundsLds = 1024;
+
+__device__ __forceinline__ void stage_bounds(const GlPassArgs &A, int *s_bnd) {
+    if (A.n_groups <= kGlBoundsLds)
+        for (int g = threadIdx.x; g <= A.n_groups; g += kBlock) s_bnd[g] = A.bounds[g];
+    __syncthreads();
+}
+
 // Fused GL-DROT pass for one tile of `tc` whole columns.  Restates
 // group_lasso_regularizer_update_x_for_small (core/kernel_glr.hpp:59-161) and
 // the large-group chain update_x / update_norm / apply_norm
@@ -113,13 +134,33 @@ template <int VEC, int RCH>
 __global__ __launch_bounds__(kBlock) void gl_pass_kernel(GlPassArgs A) {
     extern __shared__ double s_dyn[];
     __shared__ double s_red[3 * kWavesPerBlock];
+    __shared__ int s_bnd[kGlBoundsLds + 1];
     const unsigned long long t0 = A.ctl.tspan ? (unsigned long long)wall_clock64() : 0ull;
     if (A.ctl.st->done) return;
+    stage_bounds(A, s_bnd);
+    // the rows' phi numerators and groups are loaded before the prologue's
+    // reductions, so their round trip overlaps it
+    const bool explicit_phi = A.ctl.k == 0;
+    const float *hsrc = explicit_phi ? A.phi1 : A.ahat;
+    float h1[RCH][VEC];
+    int gid[RCH][VEC];
+#pragma unroll
+    for (int r = 0; r < RCH; r++) {
+        const int i0 = (r * kBlock + (int)threadIdx.x) * VEC;
+#pragma unroll
+        for (int e = 0; e < VEC; e++) { h1[r][e] = 0.0f; gid[r][e] = 0; }
+        if (i0 < A.n_rows) {
+            VecIO<VEC>::load(hsrc + i0, h1[r]);
+            load_ints<VEC>(A.row_group + i0, gid[r]);
+        }
+    }
     const IterStart is = iteration_prologue(A.ctl, s_red, blockIdx.x == 0);
     if (is.finish) return;
     const int phi_mode = is.explicit_phi ? 1 : 0;
     const float gamma = is.gamma;
     const int N = A.n_rows, M = A.n_cols, G = A.n_groups;
+    const bool bnd_lds = G <= kGlBoundsLds;
+    auto bnd = [&](int g) { return bnd_lds ? s_bnd[g] : A.bounds[g]; };
     double *s_sq = s_dyn;                                   // [N]
     float *s_scale = reinterpret_cast<float *>(s_dyn + N);  // [G]
     const float fM = (float)M, fN = (float)N;
@@ -129,26 +170,13 @@ __global__ __launch_bounds__(kBlock) void gl_pass_kernel(GlPassArgs A) {
     const int w = wave_uniform(threadIdx.x / kWave);
 
     float f1[RCH][VEC];
-    int gid[RCH][VEC];
     double racc[RCH][VEC];
 #pragma unroll
     for (int r = 0; r < RCH; r++) {
 #pragma unroll
         for (int e = 0; e < VEC; e++) {
-            const int i = (r * kBlock + (int)threadIdx.x) * VEC + e;
             racc[r][e] = 0.0;
-            f1[r][e] = 0.0f;
-            gid[r][e] = 0;
-            if (i < N) {
-                f1[r][e] = phi_of(phi_mode, A.phi1, A.ahat, i, gamma, fM);
-                // group of row i: largest g with bounds[g] <= i
-                int lo = 0, hi = G - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (A.bounds[mid] <= i) lo = mid; else hi = mid - 1;
-                }
-                gid[r][e] = lo;
-            }
+            f1[r][e] = phi_mode ? h1[r][e] : (h1[r][e] + gamma) / fM;
         }
     }
     double oacc = 0.0;
@@ -187,14 +215,14 @@ __global__ __launch_bounds__(kBlock) void gl_pass_kernel(GlPassArgs A) {
         if (A.lane_per_group) {
             for (int g = threadIdx.x; g < G; g += kBlock) {
                 double n2 = 0.0;
-                for (int i = A.bounds[g]; i < A.bounds[g + 1]; i++) n2 += s_sq[i];
+                for (int i = bnd(g); i < bnd(g + 1); i++) n2 += s_sq[i];
                 const double sc = 1.0 - lam / (sqrt(n2) + 1e-9);
                 s_scale[g] = (float)(sc > 0.0 ? sc : 0.0);
             }
         } else {
             for (int g = w; g < G; g += kWavesPerBlock) {
                 double n2 = 0.0;
-                for (int i = A.bounds[g] + lane; i < A.bounds[g + 1]; i += kWave) n2 += s_sq[i];
+                for (int i = bnd(g) + lane; i < bnd(g + 1); i += kWave) n2 += s_sq[i];
                 n2 = wave_sum(n2);
                 if (lane == 0) {
                     const double sc = 1.0 - lam / (sqrt(n2) + 1e-9);
@@ -230,6 +258,171 @@ __global__ __launch_bounds__(kBlock) void gl_pass_kernel(GlPassArgs A) {
             const int i = (r * kBlock + (int)threadIdx.x) * VEC + e;
             if (i < N) A.prow[(size_t)blockIdx.x * N + i] = racc[r][e];
         }
+    }
+    const double o = block_sum(oacc, s_red);
+    if (threadIdx.x == 0) A.pobj[blockIdx.x] = o;
+    if (A.ctl.tspan) record_span(A.ctl.tspan, t0);
+}
+
+// GL pass with one wave per column (n_rows <= 2048): the 4 waves of a block
+// take different columns at once, so a block never waits for a column-wide
+// barrier.  A wave keeps its column's prox values in registers and a copy in
+// its own LDS slice, forms the group norms from that slice (one wave per group
+// when groups have >= 64 rows, one lane per group otherwise, fp64 in a fixed
+// order), and reads back the group scales by row.  Column sums are wave sums;
+// the row partials of the block's columns are combined across the waves
+// through LDS at the end, in wave order.  Same arithmetic as gl_pass_kernel.
+
+template <int VEC, int RCH>
+__global__ __launch_bounds__(kBlock) void gl_wave_kernel(GlPassArgs A) {
+    constexpr int kChunk = kWave * VEC;  // rows per register chunk
+    extern __shared__ float s_wave[];    // per wave: y[n_rows], scale[n_groups]
+    __shared__ double s_red[3 * kWavesPerBlock];
+    __shared__ double s_comb[kWavesPerBlock][kChunk];
+    __shared__ int s_bnd[kGlBoundsLds + 1];
+    const unsigned long long t0 = A.ctl.tspan ? (unsigned long long)wall_clock64() : 0ull;
+    if (A.ctl.st->done) return;
+    stage_bounds(A, s_bnd);
+    // the rows' phi numerators and groups are loaded before the prologue's
+    // reductions, so their round trip overlaps it
+    const int lane0 = threadIdx.x & (kWave - 1);
+    const bool explicit_phi = A.ctl.k == 0;
+    const float *hsrc = explicit_phi ? A.phi1 : A.ahat;
+    float h1[RCH][VEC];
+    int gid[RCH][VEC];
+#pragma unroll
+    for (int r = 0; r < RCH; r++) {
+        const int i0 = (r * kWave + lane0) * VEC;
+#pragma unroll
+        for (int e = 0; e < VEC; e++) { h1[r][e] = 0.0f; gid[r][e] = 0; }
+        if (i0 < A.n_rows) {
+            VecIO<VEC>::load(hsrc + i0, h1[r]);
+            load_ints<VEC>(A.row_group + i0, gid[r]);
+        }
+    }
+    const IterStart is = iteration_prologue(A.ctl, s_red, blockIdx.x == 0);
+    if (is.finish) return;
+    const int phi_mode = is.explicit_phi ? 1 : 0;
+    const float gamma = is.gamma;
+    const int N = A.n_rows, M = A.n_cols, G = A.n_groups;
+    const bool bnd_lds = G <= kGlBoundsLds;
+    auto bnd = [&](int g) { return bnd_lds ? s_bnd[g] : A.bounds[g]; };
+    const float fM = (float)M, fN = (float)N;
+    const float s = A.step;
+    const double lam = (double)A.lambda;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = wave_uniform(threadIdx.x / kWave);
+    float *s_y = s_wave + (size_t)w * (N + G);
+    float *s_sc = s_y + N;
+
+    float f1[RCH][VEC];
+    double racc[RCH][VEC];
+#pragma unroll
+    for (int r = 0; r < RCH; r++) {
+#pragma unroll
+        for (int e = 0; e < VEC; e++) {
+            racc[r][e] = 0.0;
+            f1[r][e] = phi_mode ? h1[r][e] : (h1[r][e] + gamma) / fM;
+        }
+    }
+    double oacc = 0.0;
+    // this block's columns: [j0, j1); wave w takes j0 + w, j0 + w + 4, ...
+    const int j0 = blockIdx.x * A.tc;
+    const int j1 = min(j0 + A.tc, M);
+    for (int j = j0 + w; j < j1; j += kWavesPerBlock) {
+        const float f2 = phi_of(phi_mode, A.phi2, A.bhat, j, gamma, fN);
+        const float *xc = A.x + (size_t)j * N;
+        const float *cc = A.c + (size_t)j * N;
+        float xv[RCH][VEC], cv[RCH][VEC];
+#pragma unroll
+        for (int r = 0; r < RCH; r++) {
+            const int i0 = (r * kWave + lane) * VEC;
+            if (i0 < N) {
+                VecIO<VEC>::load(xc + i0, xv[r]);
+                VecIO<VEC>::load(cc + i0, cv[r]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < VEC; e++) { xv[r][e] = 0.0f; cv[r][e] = 0.0f; }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RCH; r++) {
+            const int i0 = (r * kWave + lane) * VEC;
+#pragma unroll
+            for (int e = 0; e < VEC; e++) {
+                const float tt = (xv[r][e] + f1[r][e]) + f2;
+                const float y = fmaxf(__builtin_fmaf(-s, cv[r][e], tt), 0.0f);
+                xv[r][e] = y;
+                if (i0 + e < N) s_y[i0 + e] = y;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // group norms -> scales (fixed order), in this wave's LDS slice
+        if (A.lane_per_group) {
+            for (int g = lane; g < G; g += kWave) {
+                double n2 = 0.0;
+                for (int i = bnd(g); i < bnd(g + 1); i++)
+                    n2 += (double)s_y[i] * (double)s_y[i];
+                const double sc = 1.0 - lam / (sqrt(n2) + 1e-9);
+                s_sc[g] = (float)(sc > 0.0 ? sc : 0.0);
+            }
+        } else {
+            for (int g = 0; g < G; g++) {
+                double n2 = 0.0;
+                for (int i = bnd(g) + lane; i < bnd(g + 1); i += kWave)
+                    n2 += (double)s_y[i] * (double)s_y[i];
+                n2 = wave_sum(n2);
+                if (lane == 0) {
+                    const double sc = 1.0 - lam / (sqrt(n2) + 1e-9);
+                    s_sc[g] = (float)(sc > 0.0 ? sc : 0.0);
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double csum = 0.0;
+#pragma unroll
+        for (int r = 0; r < RCH; r++) {
+            const int i0 = (r * kWave + lane) * VEC;
+            if (i0 < N) {
+                float xo[VEC];
+#pragma unroll
+                for (int e = 0; e < VEC; e++) {
+                    const float v = xv[r][e] * s_sc[gid[r][e]];
+                    xo[e] = v;
+                    racc[r][e] += (double)v;
+                    csum += (double)v;
+                    oacc += (double)v * (double)cv[r][e];
+                }
+                VecIO<VEC>::store(A.x + (size_t)j * N + i0, xo);
+            }
+        }
+        csum = wave_sum(csum);
+        if (lane == 0) A.pcol[j] = csum;
+        // the next column rewrites s_y only after every lane has read the scales
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // row partials of the block: waves combined in a fixed order, chunk by chunk
+#pragma unroll
+    for (int r = 0; r < RCH; r++) {
+#pragma unroll
+        for (int e = 0; e < VEC; e++) s_comb[w][lane * VEC + e] = racc[r][e];
+        __syncthreads();
+        for (int t = threadIdx.x; t < kChunk; t += kBlock) {
+            const int i = r * kChunk + t;
+            if (i < N) {
+                double sum = 0.0;
+#pragma unroll
+                for (int q = 0; q < kWavesPerBlock; q++) sum += s_comb[q][t];
+                A.prow[(size_t)blockIdx.x * N + i] = sum;
+            }
+        }
+        __syncthreads();
     }
     const double o = block_sum(oacc, s_red);
     if (threadIdx.x == 0) A.pobj[blockIdx.x] = o;
@@ -330,6 +523,21 @@ bool gl_tile_supported(const GlTile &t) {
 
 hipError_t launch_gl_pass(const GlTile &t, int grid, size_t lds, const GlPassArgs &a,
                           hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    if (t.wave) {
+#define DROT_GLW_CASE(V, R)                                                               \
+        if (t.vec == V && t.rch == R) {                                                   \
+            if (e0 && e1)                                                                 \
+                hipExtLaunchKernelGGL((gl_wave_kernel<V, R>), dim3(grid), dim3(kBlock), lds, s, \
+                                      e0, e1, 0, a);                                      \
+            else                                                                          \
+                hipLaunchKernelGGL((gl_wave_kernel<V, R>), dim3(grid), dim3(kBlock), lds, s, a); \
+            return hipGetLastError();                                                     \
+        }
+        DROT_GLW_CASE(4, 1) DROT_GLW_CASE(4, 2) DROT_GLW_CASE(4, 4) DROT_GLW_CASE(4, 8)
+        DROT_GLW_CASE(1, 1) DROT_GLW_CASE(1, 2) DROT_GLW_CASE(1, 4) DROT_GLW_CASE(1, 8)
+#undef DROT_GLW_CASE
+        return hipErrorInvalidValue;
+    }
 #define DROT_GL_CASE(V, R)                                                                \
     if (t.vec == V && t.rch == R) {                                                       \
         if (e0 && e1)                                                                     \

@@ -57,7 +57,7 @@ struct Plan {
     int kind = kQR;
     int n_rows = 0, n_cols = 0, n_groups = 1;
     QrTile qt{4, 32};
-    GlTile gt{4, 1};
+    GlTile gt{4, 1, 0};
     int n_rt = 1, n_ct = 1, nstep = 1, tc = 1;
     int grid_pass = 1, n_obj = 1;
     int rb = 1, cb = 1;
@@ -65,7 +65,7 @@ struct Plan {
     size_t lds = 0;
     size_t off_state = 0, off_phi1 = 0, off_phi2 = 0, off_a = 0, off_b = 0;
     size_t off_ahat = 0, off_bhat = 0, off_prow = 0, off_pcol = 0, off_pobj = 0;
-    size_t off_pfin = 0, off_bounds = 0, total = 0;
+    size_t off_pfin = 0, off_bounds = 0, off_rowgroup = 0, total = 0;
 };
 
 // Optional override of the QR tile, "vec,lpc,nstep[,nt[,u]]" (tuning sweeps only).
@@ -103,6 +103,7 @@ void layout(Plan &P) {
     P.off_pobj = take((size_t)P.n_obj * 8);
     P.off_pfin = take((size_t)(P.rb + P.cb) * 24);
     P.off_bounds = take((size_t)(P.n_groups + 1) * 4);
+    P.off_rowgroup = take(P.kind == kGL ? N * 4 : 0);
     P.total = off;
 }
 
@@ -205,6 +206,33 @@ bool plan_gl(int n_rows, int n_cols, int n_groups, bool vec4_ok, Plan *out,
     P.n_groups = n_groups;
     const bool v4 = vec4_ok && (n_rows % 4 == 0);
     P.gt.vec = v4 ? 4 : 1;
+    // one wave per column when a column fits a wave's registers and the
+    // per-wave LDS slices fit (DROT_GL_WAVE=0 selects the block-per-column
+    // kernel, for tests that compare the two)
+    const char *wave_env = getenv("DROT_GL_WAVE");
+    const bool wave_ok = !(wave_env && wave_env[0] == '0');
+    const int wchunk = kWave * P.gt.vec;
+    int wrch = 1;
+    while (wrch < 8 && wchunk * wrch < n_rows) wrch *= 2;
+    const size_t wave_lds = (size_t)kWavesPerBlock * ((size_t)n_rows + n_groups) * 4;
+    if (wave_ok && wchunk * wrch >= n_rows && wave_lds <= 48 * 1024) {
+        P.gt.wave = 1;
+        P.gt.rch = wrch;
+        P.lds = wave_lds;
+        // about 256 blocks: row partials cost 8 B per row per block
+        const int cpw = std::max(1, ceil_div(n_cols, kWavesPerBlock * 256));
+        P.tc = kWavesPerBlock * cpw;
+        P.n_ct = ceil_div(n_cols, P.tc);
+        P.n_rt = 1;
+        P.grid_pass = P.n_ct;
+        P.n_obj = P.n_ct;
+        P.lane_per_group = (n_rows / std::max(1, n_groups)) < 64 ? 1 : 0;
+        P.rb = ceil_div(n_rows, kFinRows);
+        P.cb = ceil_div(n_cols, kFinRows);
+        layout(P);
+        *out = P;
+        return true;
+    }
     const int maxr = v4 ? 8 : 32;
     int rch = 1;
     while (rch < maxr && kBlock * P.gt.vec * rch < n_rows) rch *= 2;
@@ -287,6 +315,7 @@ struct DevPtrs {
     float *phi1, *phi2, *a, *b, *ahat, *bhat;
     double *prow, *pcol, *pobj, *pfin;
     int *bounds;
+    int *row_group;
 };
 
 DevPtrs carve(const Plan &P, void *ws) {
@@ -304,6 +333,7 @@ DevPtrs carve(const Plan &P, void *ws) {
     d.pobj = reinterpret_cast<double *>(base + P.off_pobj);
     d.pfin = reinterpret_cast<double *>(base + P.off_pfin);
     d.bounds = reinterpret_cast<int *>(base + P.off_bounds);
+    d.row_group = reinterpret_cast<int *>(base + P.off_rowgroup);
     return d;
 }
 
@@ -409,7 +439,12 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
         for (int g = 0; g <= G; g++) hb[g] = (int)((double)(per * float(g)) + off);
         if (hb[0] != 0 || hb[G] != P.n_rows)
             return fail(DROT_ERR_UNSUPPORTED, "group bounds do not cover all rows");
+        std::vector<int> rg((size_t)P.n_rows);  // group of each row
+        for (int g = 0; g < G; g++)
+            for (int i = hb[g]; i < hb[g + 1]; i++) rg[(size_t)i] = g;
         DROT_HIP_CHECK(hipMemcpyAsync(d.bounds, hb.data(), sizeof(int) * (G + 1),
+                                      hipMemcpyHostToDevice, stream));
+        DROT_HIP_CHECK(hipMemcpyAsync(d.row_group, rg.data(), sizeof(int) * rg.size(),
                                       hipMemcpyHostToDevice, stream));
         DROT_HIP_CHECK(hipStreamSynchronize(stream));
         ga.x = in.x;
@@ -422,6 +457,7 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
         ga.pcol = d.pcol;
         ga.pobj = d.pobj;
         ga.bounds = d.bounds;
+        ga.row_group = d.row_group;
         ga.n_rows = P.n_rows;
         ga.n_cols = P.n_cols;
         ga.n_groups = P.n_groups;
