@@ -115,6 +115,7 @@ __global__ __launch_bounds__(kBlock) void batch_pass_kernel(QrBatchArgs A) {
     J.rt = bt.rt;
     J.ct = bt.ct;
     J.nstep = P.nstep;
+    J.prio = 0;
     J.tc = P.nstep * kWavesPerBlock * CPW;
     J.step = P.step;
     J.kappa = P.kappa;

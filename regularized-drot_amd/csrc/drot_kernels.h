@@ -21,6 +21,7 @@ struct QrPassArgs {
     int n_rows, n_cols;
     int n_rt, n_ct;            // row tiles, column tiles
     int nstep;                 // column steps per wave; tile columns = nstep * 4 * (64 / lpc)
+    int prio;                  // set by launch_qr_pass: 1 = issue priority by remaining steps
     float step, kappa;
     IterCtl ctl;
 };

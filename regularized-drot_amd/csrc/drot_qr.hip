@@ -47,6 +47,7 @@ __global__ __launch_bounds__(kBlock) void qr_pass_kernel(QrPassArgs A) {
     J.rt = tile % n_rt;
     J.ct = tile / n_rt;
     J.nstep = A.nstep;
+    J.prio = A.prio;
     J.tc = A.nstep * kWavesPerBlock * CPW;
     J.step = A.step;
     J.kappa = A.kappa;
@@ -61,14 +62,32 @@ bool qr_tile_supported(const QrTile &t) {
            (t.nt == 0 || t.nt == 2) && (t.u == 2 || t.u == 4);
 }
 
+// Blocks of kernel k the device holds at once (0 when the query fails).
+static int grid_capacity(const void *k) {
+    int dev = 0, n_cu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, 0) != hipSuccess)
+        return 0;
+    return n_cu * per_cu;
+}
+
 template <int VEC, int LPC, int ST, int U>
 static void launch_pass_t(bool even, int grid, const QrPassArgs &a, hipStream_t s,
                           hipEvent_t e0, hipEvent_t e1) {
     auto k = even ? qr_pass_kernel<VEC, LPC, true, ST, U> : qr_pass_kernel<VEC, LPC, false, ST, U>;
+    static const int cap_even =
+        grid_capacity(reinterpret_cast<const void *>(qr_pass_kernel<VEC, LPC, true, ST, U>));
+    static const int cap_odd =
+        grid_capacity(reinterpret_cast<const void *>(qr_pass_kernel<VEC, LPC, false, ST, U>));
+    // ranking waves by remaining work pays only when the whole grid is resident
+    // (one round of blocks); with later rounds it costs a few percent
+    QrPassArgs b = a;
+    b.prio = grid <= (even ? cap_even : cap_odd) ? 1 : 0;
     if (e0 && e1)  // timing: events bracket the kernel's own execution
-        hipExtLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, a);
+        hipExtLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, b);
     else
-        hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, b);
 }
 
 template <int VEC, int LPC>

@@ -78,8 +78,38 @@ struct QrTileJob {
     double *pobj;   // this tile's objective partial
     int n_rows, n_cols, ld;
     int rt, ct, tc, nstep;
+    int prio;       // 1: waves raise their issue priority while many column steps remain
     float step, kappa;
 };
+
+// y / kappa, correctly rounded, from the correctly rounded reciprocal
+// rkappa = RN(1/kappa): q0 = RN(y * rkappa) is within one ulp of y/kappa, the
+// remainder y - kappa*q0 is then exact in one fma, and RN(q0 + rem*rkappa) is
+// the correctly rounded quotient (Markstein's theorem, round-to-nearest).
+// The remainder is exact only while it does not underflow, so quotients below
+// 2^-90 (and above 2^120, or not finite) go through the true division.
+__device__ __forceinline__ float div_kappa(float y, float kappa, float rkappa) {
+    const float q0 = y * rkappa;
+    const float rem = __builtin_fmaf(-q0, kappa, y);
+    return __builtin_fmaf(rem, rkappa, q0);
+}
+
+__device__ __forceinline__ bool div_kappa_slow(float y, float q) {
+    return y > 0.0f && !(q >= 0x1p-90f && q <= 0x1p+120f);
+}
+
+// Issue priority from the share of column steps still ahead (3 at the start,
+// 0 for the last batch).  When every tile of a launch is resident at once,
+// the SIMD arbiter favours the oldest waves, so the blocks dispatched last on
+// a CU fall behind and finish alone; ranking waves by remaining work keeps the
+// blocks of the launch in step and shortens that tail.
+__device__ __forceinline__ void set_issue_priority(int remaining, int nstep) {
+    const int pr = min(3, max(0, (4 * remaining - 1) / nstep));
+    if (pr == 3) __builtin_amdgcn_s_setprio(3);
+    else if (pr == 2) __builtin_amdgcn_s_setprio(2);
+    else if (pr == 1) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+}
 
 template <int LPC>
 struct QrTileShared {
@@ -145,6 +175,7 @@ __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, 
     if (is.finish) return;
     const float gamma = is.gamma;
     const float s = J.step, kappa = J.kappa;
+    const float rkappa = 1.0f / kappa;
 
     float f1[VEC];
 #pragma unroll
@@ -155,6 +186,7 @@ __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, 
     double oacc = 0.0;
 
     for (int t0 = 0; t0 < nstep; t0 += U) {
+        if (J.prio) set_issue_priority(nstep - t0, nstep);
         float xb[U][VEC], cb[U][VEC], h2b[U];
         if (t0 + U < nstep) load_batch(t0 + U, xb, cb, h2b);
 #pragma unroll
@@ -165,14 +197,27 @@ __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, 
             const bool col_ok = j < M;
             const bool ok = col_ok && row_ok;
             const float f2 = explicit_phi ? h2a[u] : (h2a[u] + gamma) / fN;
-            float xo[VEC];
+            float xo[VEC], yv[VEC], xq[VEC];
             double csum = 0.0;
+            bool slow = false;
 #pragma unroll
             for (int e = 0; e < VEC; e++) {
                 const float tt = (xa[u][e] + f1[e]) + f2;
                 float y = EVEN ? __builtin_fmaf(-s, ca[u][e], tt) : tt;
-                y = fmaxf(y, 0.0f);
-                const float xp = y / kappa;
+                yv[e] = fmaxf(y, 0.0f);
+                xq[e] = div_kappa(yv[e], kappa, rkappa);
+                slow |= div_kappa_slow(yv[e], xq[e]);
+            }
+            // true division for quotients outside the range where the
+            // correction step is exact (wave-uniform branch, normally skipped)
+            if (__builtin_amdgcn_ballot_w64(slow)) {
+#pragma unroll
+                for (int e = 0; e < VEC; e++)
+                    if (div_kappa_slow(yv[e], xq[e])) xq[e] = yv[e] / kappa;
+            }
+#pragma unroll
+            for (int e = 0; e < VEC; e++) {
+                const float xp = xq[e];
                 xo[e] = EVEN ? __builtin_fmaf(-s, ca[u][e], xp) : xp;
                 if (ok && ev_ok[e]) {
                     racc[e] += (double)xp;

@@ -134,3 +134,24 @@ def test_qr_large_first_iterations(device, n):
     ref = O.qr_solve(c, p, q, np.float32(prm.step_size), np.float32(prm.scale), 6, 0.0)
     assert rel_err(x, ref.x) <= PLAN_RTOL
     assert rel_err(phi1, ref.state.phi1) <= PLAN_RTOL
+
+
+@pytest.mark.parametrize("r_weight", [1e-4, 1e-1, 30.0])
+def test_qr_first_iteration_bit_exact(device, r_weight):
+    """The first iteration involves no reduction (the warm start fixes phi), so
+    its element arithmetic must give the oracle's bits exactly: the prox
+    max(X + phi1 + phi2 - sC, 0)/scale with a correctly rounded division, then
+    the stored X' - sC.  Costs of both signs spanning 1e-3..3e38 drive the
+    quotients over the whole exponent range, including above 2^120."""
+    n_rows, n_cols = 256, 96
+    rng = np.random.default_rng(11)
+    mag = 10.0 ** rng.uniform(-3.0, 38.5, size=(n_cols, n_rows))
+    c = (mag * rng.choice([-1.0, 1.0], size=mag.shape)).astype(np.float32)
+    p, q = problems.uniform_marginals(n_rows, n_cols)
+    prm = _abi().qr_params_from_rho(2.0, r_weight, n_rows, n_cols, 1, 0.0)
+    x, r, *_ = gpu_qr(c, p, q, prm, device)
+    ref = O.qr_solve(c, p, q, np.float32(prm.step_size), np.float32(prm.scale), 1, 0.0)
+    assert r.iterations_run == 1
+    want = np.asarray(ref.x, dtype=np.float32)
+    assert np.isfinite(want).all()
+    assert np.array_equal(x.view(np.uint32), want.view(np.uint32))
