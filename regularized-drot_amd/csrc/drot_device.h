@@ -76,12 +76,30 @@ __device__ __forceinline__ void record_span(unsigned long long *tspan, unsigned 
 
 // ---------------------------------------------------------------- reductions
 
-// Butterfly sum over aligned segments of `width` lanes (width a power of two,
-// <= 64).  Every lane of a segment ends with the same, order-fixed result.
+// A double moved across lanes by a DPP control (two 32-bit DPP moves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+// Butterfly sum over aligned segments of `width` lanes (16, 32 or 64).  Lane
+// i adds lane i ^ off for off = width/2, ..., 2, 1; every lane of a segment
+// ends with the same, order-fixed result.  Offsets 32 and 16 cross 16-lane
+// rows (a bpermute through LDS); the rest stay in registers: row_ror:8 is
+// xor 8 within a row, row_ror:4 equals xor 4 once the values repeat every 8
+// lanes (after the xor-8 step), and quad_perm gives xor 2 and xor 1.
 template <int WIDTH>
 __device__ __forceinline__ double seg_sum(double v) {
-#pragma unroll
-    for (int off = WIDTH / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+    static_assert(WIDTH == 16 || WIDTH == 32 || WIDTH == 64, "segment of 16, 32 or 64 lanes");
+    if constexpr (WIDTH >= 64) v += __shfl_xor(v, 32, kWave);
+    if constexpr (WIDTH >= 32) v += __shfl_xor(v, 16, kWave);
+    v += dpp_move<0x128>(v);  // row_ror:8
+    v += dpp_move<0x124>(v);  // row_ror:4
+    v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
     return v;
 }
 

@@ -110,6 +110,15 @@ __global__ __launch_bounds__(kBlock) void init_kernel(InitArgs A) {
 
 // ----------------------------------------------------------------- GL pass
 
+// Soft-threshold scale of one group column, core/kernel_glr.hpp:125,320:
+// max(1 - lambda / (||x||_2 + 1e-9), 0), from the fp64 sum of squares.
+__device__ __forceinline__ float group_scale(double n2, double lam) {
+    const double sc = 1.0 - lam / (sqrt(n2) + 1e-9);
+    return (float)(sc > 0.0 ? sc : 0.0);
+}
+
+constexpr int kNormBatch = 8;  // LDS reads in flight per lane in the group norms
+
 // Group row bounds are read from LDS when there are at most kGlBoundsLds
 // groups (copied at kernel start).
 constexpr int kGlBoundsLds = 1024;
@@ -212,22 +221,38 @@ __global__ __launch_bounds__(kBlock) void gl_pass_kernel(GlPassArgs A) {
         }
         __syncthreads();
         // group norms -> scales, fixed order
+        // (LDS reads are issued kNormBatch at a time; the additions keep their
+        // sequential order)
         if (A.lane_per_group) {
             for (int g = threadIdx.x; g < G; g += kBlock) {
+                const int hi = bnd(g + 1);
+                int i = bnd(g);
                 double n2 = 0.0;
-                for (int i = bnd(g); i < bnd(g + 1); i++) n2 += s_sq[i];
-                const double sc = 1.0 - lam / (sqrt(n2) + 1e-9);
-                s_scale[g] = (float)(sc > 0.0 ? sc : 0.0);
+                for (; i + kNormBatch <= hi; i += kNormBatch) {
+                    double v[kNormBatch];
+#pragma unroll
+                    for (int u = 0; u < kNormBatch; u++) v[u] = s_sq[i + u];
+#pragma unroll
+                    for (int u = 0; u < kNormBatch; u++) n2 += v[u];
+                }
+                for (; i < hi; i++) n2 += s_sq[i];
+                s_scale[g] = group_scale(n2, lam);
             }
         } else {
             for (int g = w; g < G; g += kWavesPerBlock) {
+                const int hi = bnd(g + 1);
+                int i = bnd(g) + lane;
                 double n2 = 0.0;
-                for (int i = bnd(g) + lane; i < bnd(g + 1); i += kWave) n2 += s_sq[i];
-                n2 = wave_sum(n2);
-                if (lane == 0) {
-                    const double sc = 1.0 - lam / (sqrt(n2) + 1e-9);
-                    s_scale[g] = (float)(sc > 0.0 ? sc : 0.0);
+                for (; i + (kNormBatch - 1) * kWave < hi; i += kNormBatch * kWave) {
+                    double v[kNormBatch];
+#pragma unroll
+                    for (int u = 0; u < kNormBatch; u++) v[u] = s_sq[i + u * kWave];
+#pragma unroll
+                    for (int u = 0; u < kNormBatch; u++) n2 += v[u];
                 }
+                for (; i < hi; i += kWave) n2 += s_sq[i];
+                n2 = wave_sum(n2);
+                if (lane == 0) s_scale[g] = group_scale(n2, lam);
             }
         }
         __syncthreads();
@@ -278,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void gl_wave_kernel(GlPassArgs A) {
     constexpr int kChunk = kWave * VEC;  // rows per register chunk
     extern __shared__ float s_wave[];    // per wave: y[n_rows], scale[n_groups]
     __shared__ double s_red[3 * kWavesPerBlock];
-    __shared__ double s_comb[kWavesPerBlock][kChunk];
+    __shared__ double s_comb[2][kWavesPerBlock][kChunk];
     __shared__ int s_bnd[kGlBoundsLds + 1];
     const unsigned long long t0 = A.ctl.tspan ? (unsigned long long)wall_clock64() : 0ull;
     if (A.ctl.st->done) return;
@@ -300,6 +325,30 @@ __global__ __launch_bounds__(kBlock) void gl_wave_kernel(GlPassArgs A) {
             load_ints<VEC>(A.row_group + i0, gid[r]);
         }
     }
+    // this block's columns: [j0, j1); wave w takes j0 + w, j0 + w + 4, ...
+    // The first column's X and C are loaded before the prologue as well.
+    const int w = wave_uniform(threadIdx.x / kWave);
+    const int j0 = blockIdx.x * A.tc;
+    const int j1 = min(j0 + A.tc, A.n_cols);
+    float xv[RCH][VEC], cv[RCH][VEC];
+    float h2 = 0.0f;
+    auto load_col = [&](int j) {
+        const float *xc = A.x + (size_t)j * A.n_rows;
+        const float *cc = A.c + (size_t)j * A.n_rows;
+        h2 = explicit_phi ? A.phi2[j] : A.bhat[j];
+#pragma unroll
+        for (int r = 0; r < RCH; r++) {
+            const int i0 = (r * kWave + lane0) * VEC;
+            if (i0 < A.n_rows) {
+                VecIO<VEC>::load(xc + i0, xv[r]);
+                VecIO<VEC>::load(cc + i0, cv[r]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < VEC; e++) { xv[r][e] = 0.0f; cv[r][e] = 0.0f; }
+            }
+        }
+    };
+    if (j0 + w < j1) load_col(j0 + w);
     const IterStart is = iteration_prologue(A.ctl, s_red, blockIdx.x == 0);
     if (is.finish) return;
     const int phi_mode = is.explicit_phi ? 1 : 0;
@@ -310,8 +359,7 @@ __global__ __launch_bounds__(kBlock) void gl_wave_kernel(GlPassArgs A) {
     const float fM = (float)M, fN = (float)N;
     const float s = A.step;
     const double lam = (double)A.lambda;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int w = wave_uniform(threadIdx.x / kWave);
+    const int lane = lane0;
     float *s_y = s_wave + (size_t)w * (N + G);
     float *s_sc = s_y + N;
 
@@ -326,25 +374,9 @@ __global__ __launch_bounds__(kBlock) void gl_wave_kernel(GlPassArgs A) {
         }
     }
     double oacc = 0.0;
-    // this block's columns: [j0, j1); wave w takes j0 + w, j0 + w + 4, ...
-    const int j0 = blockIdx.x * A.tc;
-    const int j1 = min(j0 + A.tc, M);
     for (int j = j0 + w; j < j1; j += kWavesPerBlock) {
-        const float f2 = phi_of(phi_mode, A.phi2, A.bhat, j, gamma, fN);
-        const float *xc = A.x + (size_t)j * N;
-        const float *cc = A.c + (size_t)j * N;
-        float xv[RCH][VEC], cv[RCH][VEC];
-#pragma unroll
-        for (int r = 0; r < RCH; r++) {
-            const int i0 = (r * kWave + lane) * VEC;
-            if (i0 < N) {
-                VecIO<VEC>::load(xc + i0, xv[r]);
-                VecIO<VEC>::load(cc + i0, cv[r]);
-            } else {
-#pragma unroll
-                for (int e = 0; e < VEC; e++) { xv[r][e] = 0.0f; cv[r][e] = 0.0f; }
-            }
-        }
+        if (j != j0 + w) load_col(j);
+        const float f2 = phi_mode ? h2 : (h2 + gamma) / fN;
 #pragma unroll
         for (int r = 0; r < RCH; r++) {
             const int i0 = (r * kWave + lane) * VEC;
@@ -360,23 +392,44 @@ __global__ __launch_bounds__(kBlock) void gl_wave_kernel(GlPassArgs A) {
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // group norms -> scales (fixed order), in this wave's LDS slice
+        // (LDS reads are issued kNormBatch at a time; the additions keep their
+        // sequential order)
         if (A.lane_per_group) {
             for (int g = lane; g < G; g += kWave) {
+                const int hi = bnd(g + 1);
+                int i = bnd(g);
                 double n2 = 0.0;
-                for (int i = bnd(g); i < bnd(g + 1); i++)
-                    n2 += (double)s_y[i] * (double)s_y[i];
-                const double sc = 1.0 - lam / (sqrt(n2) + 1e-9);
-                s_sc[g] = (float)(sc > 0.0 ? sc : 0.0);
+                for (; i + kNormBatch <= hi; i += kNormBatch) {
+                    float v[kNormBatch];
+#pragma unroll
+                    for (int u = 0; u < kNormBatch; u++) v[u] = s_y[i + u];
+#pragma unroll
+                    for (int u = 0; u < kNormBatch; u++) n2 += (double)v[u] * (double)v[u];
+                }
+                for (; i < hi; i++) n2 += (double)s_y[i] * (double)s_y[i];
+                s_sc[g] = group_scale(n2, lam);
             }
         } else {
+            // wave-wide sums per group; lane g % 64 keeps group g's total and
+            // the scales of up to 64 groups are formed at once
+            double mine = 0.0;
             for (int g = 0; g < G; g++) {
+                const int hi = bnd(g + 1);
+                int i = bnd(g) + lane;
                 double n2 = 0.0;
-                for (int i = bnd(g) + lane; i < bnd(g + 1); i += kWave)
-                    n2 += (double)s_y[i] * (double)s_y[i];
+                for (; i + (kNormBatch - 1) * kWave < hi; i += kNormBatch * kWave) {
+                    float v[kNormBatch];
+#pragma unroll
+                    for (int u = 0; u < kNormBatch; u++) v[u] = s_y[i + u * kWave];
+#pragma unroll
+                    for (int u = 0; u < kNormBatch; u++) n2 += (double)v[u] * (double)v[u];
+                }
+                for (; i < hi; i += kWave) n2 += (double)s_y[i] * (double)s_y[i];
                 n2 = wave_sum(n2);
-                if (lane == 0) {
-                    const double sc = 1.0 - lam / (sqrt(n2) + 1e-9);
-                    s_sc[g] = (float)(sc > 0.0 ? sc : 0.0);
+                if ((g & (kWave - 1)) == lane) mine = n2;
+                if ((g & (kWave - 1)) == kWave - 1 || g == G - 1) {
+                    const int gb = g & ~(kWave - 1);
+                    if (gb + lane <= g) s_sc[gb + lane] = group_scale(mine, lam);
                 }
             }
         }
@@ -407,22 +460,26 @@ __global__ __launch_bounds__(kBlock) void gl_wave_kernel(GlPassArgs A) {
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    // row partials of the block: waves combined in a fixed order, chunk by chunk
+    // row partials of the block: waves combined in a fixed order, chunk by
+    // chunk through two alternating LDS buffers (one barrier per chunk; the
+    // barrier of chunk r + 1 orders the reads of chunk r before the writes of
+    // chunk r + 2 into the same buffer)
 #pragma unroll
     for (int r = 0; r < RCH; r++) {
+        if (r * kChunk >= N) break;
+        double (*cb)[kChunk] = s_comb[r & 1];
 #pragma unroll
-        for (int e = 0; e < VEC; e++) s_comb[w][lane * VEC + e] = racc[r][e];
+        for (int e = 0; e < VEC; e++) cb[w][lane * VEC + e] = racc[r][e];
         __syncthreads();
         for (int t = threadIdx.x; t < kChunk; t += kBlock) {
             const int i = r * kChunk + t;
             if (i < N) {
                 double sum = 0.0;
 #pragma unroll
-                for (int q = 0; q < kWavesPerBlock; q++) sum += s_comb[q][t];
+                for (int q = 0; q < kWavesPerBlock; q++) sum += cb[q][t];
                 A.prow[(size_t)blockIdx.x * N + i] = sum;
             }
         }
-        __syncthreads();
     }
     const double o = block_sum(oacc, s_red);
     if (threadIdx.x == 0) A.pobj[blockIdx.x] = o;
