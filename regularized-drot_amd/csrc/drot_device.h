@@ -85,17 +85,41 @@ __device__ __forceinline__ double dpp_move(double v) {
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
 }
 
+// v + (v of lane i ^ 32) or (lane i ^ 16) through gfx950's row swaps: with
+// both operands equal, v_permlane32_swap leaves {v_i, v_(i^32)} in the two
+// results of every lane i, v_permlane16_swap {v_i, v_(i^16)}.
+__device__ __forceinline__ double xor32_add(double v) {
+    const long long u = __double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)u, (unsigned)u, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(u >> 32), (unsigned)(u >> 32),
+                                                     false, false);
+    const double a = __longlong_as_double(((long long)hi[0] << 32) | (long long)lo[0]);
+    const double b = __longlong_as_double(((long long)hi[1] << 32) | (long long)lo[1]);
+    return a + b;
+}
+
+__device__ __forceinline__ double xor16_add(double v) {
+    const long long u = __double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)u, (unsigned)u, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(u >> 32), (unsigned)(u >> 32),
+                                                     false, false);
+    const double a = __longlong_as_double(((long long)hi[0] << 32) | (long long)lo[0]);
+    const double b = __longlong_as_double(((long long)hi[1] << 32) | (long long)lo[1]);
+    return a + b;
+}
+
 // Butterfly sum over aligned segments of `width` lanes (16, 32 or 64).  Lane
 // i adds lane i ^ off for off = width/2, ..., 2, 1; every lane of a segment
-// ends with the same, order-fixed result.  Offsets 32 and 16 cross 16-lane
-// rows (a bpermute through LDS); the rest stay in registers: row_ror:8 is
-// xor 8 within a row, row_ror:4 equals xor 4 once the values repeat every 8
-// lanes (after the xor-8 step), and quad_perm gives xor 2 and xor 1.
+// ends with the same, order-fixed result (the two operands of each addition
+// are the same pair in every lane of it, so the order inside a pair does not
+// change the bits).  All in registers: the row swaps for offsets 32 and 16,
+// row_ror:8 for xor 8 within a 16-lane row, row_ror:4 for xor 4 once the
+// values repeat every 8 lanes (after the xor-8 step), quad_perm for 2 and 1.
 template <int WIDTH>
 __device__ __forceinline__ double seg_sum(double v) {
     static_assert(WIDTH == 16 || WIDTH == 32 || WIDTH == 64, "segment of 16, 32 or 64 lanes");
-    if constexpr (WIDTH >= 64) v += __shfl_xor(v, 32, kWave);
-    if constexpr (WIDTH >= 32) v += __shfl_xor(v, 16, kWave);
+    if constexpr (WIDTH >= 64) v = xor32_add(v);
+    if constexpr (WIDTH >= 32) v = xor16_add(v);
     v += dpp_move<0x128>(v);  // row_ror:8
     v += dpp_move<0x124>(v);  // row_ror:4
     v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
