@@ -6,6 +6,8 @@
 // grid-wide steps of an iteration are the kernel boundaries pass -> finalize
 // -> next pass, whose prologue finishes the previous iteration
 // (iteration_prologue in drot_device.h).
+#include <atomic>
+
 #include "drot_qr_tile.h"
 
 #include <math.h>
@@ -62,28 +64,45 @@ bool qr_tile_supported(const QrTile &t) {
            (t.nt == 0 || t.nt == 2) && (t.u == 2 || t.u == 4);
 }
 
-// Blocks of kernel k the device holds at once (0 when the query fails).
-static int grid_capacity(const void *k) {
-    int dev = 0, n_cu = 0, per_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, 0) != hipSuccess)
-        return 0;
-    return n_cu * per_cu;
+// Blocks of kernel k the current device holds at once (0 when the query
+// fails), cached per kernel instance and device as capacity + 1 (0: not
+// queried yet; concurrent first calls store the same value).
+constexpr int kMaxDevices = 64;
+
+static int grid_capacity(const void *k, std::atomic<int> *cache) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 0;
+    std::atomic<int> *slot = dev < kMaxDevices ? &cache[dev] : nullptr;
+    if (slot) {
+        const int v = slot->load(std::memory_order_relaxed);
+        if (v > 0) return v - 1;
+    }
+    int n_cu = 0, per_cu = 0;
+    int cap = 0;
+    if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, 0) == hipSuccess)
+        cap = n_cu * per_cu;
+    if (slot) slot->store(cap + 1, std::memory_order_relaxed);
+    return cap;
+}
+
+template <bool EVEN, int VEC, int LPC, int ST, int U>
+static int pass_capacity() {
+    static std::atomic<int> cache[kMaxDevices];  // zero-initialised (static storage)
+    return grid_capacity(reinterpret_cast<const void *>(qr_pass_kernel<VEC, LPC, EVEN, ST, U>),
+                         cache);
 }
 
 template <int VEC, int LPC, int ST, int U>
 static void launch_pass_t(bool even, int grid, const QrPassArgs &a, hipStream_t s,
                           hipEvent_t e0, hipEvent_t e1) {
     auto k = even ? qr_pass_kernel<VEC, LPC, true, ST, U> : qr_pass_kernel<VEC, LPC, false, ST, U>;
-    static const int cap_even =
-        grid_capacity(reinterpret_cast<const void *>(qr_pass_kernel<VEC, LPC, true, ST, U>));
-    static const int cap_odd =
-        grid_capacity(reinterpret_cast<const void *>(qr_pass_kernel<VEC, LPC, false, ST, U>));
     // ranking waves by remaining work pays only when the whole grid is resident
     // (one round of blocks); with later rounds it costs a few percent
+    const int cap = even ? pass_capacity<true, VEC, LPC, ST, U>()
+                         : pass_capacity<false, VEC, LPC, ST, U>();
     QrPassArgs b = a;
-    b.prio = grid <= (even ? cap_even : cap_odd) ? 1 : 0;
+    b.prio = grid <= cap ? 1 : 0;
     if (e0 && e1)  // timing: events bracket the kernel's own execution
         hipExtLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, b);
     else
