@@ -145,7 +145,9 @@ __device__ __forceinline__ double block_sum(double v, double *scratch) {
 
 // Fixed-order block-wide sums of three doubles at once (one LDS round trip).
 // `scratch` holds 3 * kWavesPerBlock doubles.  Results valid in every thread.
-__device__ __forceinline__ void block_sum3(double &a, double &b, double &c, double *scratch) {
+// reuse = false drops the closing barrier, for a last use of `scratch`.
+__device__ __forceinline__ void block_sum3(double &a, double &b, double &c, double *scratch,
+                                           bool reuse = true) {
     a = wave_sum(a);
     b = wave_sum(b);
     c = wave_sum(c);
@@ -163,7 +165,7 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c, doub
         tb += scratch[kWavesPerBlock + i];
         tc += scratch[2 * kWavesPerBlock + i];
     }
-    __syncthreads();
+    if (reuse) __syncthreads();
     a = ta;
     b = tb;
     c = tc;

@@ -99,7 +99,7 @@ __device__ inline void finalize_block(const FinJob &A, double (*s_part)[kFinRows
         }
         res = d * d;
     }
-    block_sum3(res, S, obj, s_red);
+    block_sum3(res, S, obj, s_red, false);  // the block ends after these stores
     if (threadIdx.x == 0) {
         A.pfin[0] = res;
         A.pfin[1] = S;

@@ -244,10 +244,18 @@ __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, 
         }
     }
 
-    // strided-axis partials: fixed-order combine of the 4*CPW column slots
+    // strided-axis partials: fixed-order combine of the 4*CPW column slots;
+    // the objective's wave sums meet behind the same barrier (in their own LDS
+    // slots: s_red may still be read by the prologue of a slower wave), and
+    // are added in wave order as block_sum does
+    __shared__ double s_obj[kWavesPerBlock];
     const int slot = w * CPW + cs;
 #pragma unroll
     for (int e = 0; e < VEC; e++) s_row[slot][rl * VEC + e] = racc[e];
+    if (EVEN) {
+        const double ow = wave_sum(oacc);
+        if (lane == 0) s_obj[w] = ow;
+    }
     __syncthreads();
     for (int r = threadIdx.x; r < TR; r += kBlock) {
         double sum = 0.0;
@@ -256,9 +264,11 @@ __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, 
         const int i = J.rt * TR + r;
         if (i < N) J.prow[(size_t)J.ct * N + i] = sum;
     }
-    if (EVEN) {
-        const double o = block_sum(oacc, s_red);
-        if (threadIdx.x == 0) *J.pobj = o;
+    if (EVEN && threadIdx.x == 0) {
+        double o = 0.0;
+#pragma unroll
+        for (int q = 0; q < kWavesPerBlock; q++) o += s_obj[q];
+        *J.pobj = o;
     }
 }
 
