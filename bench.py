@@ -284,8 +284,8 @@ def batched_fwd_bwd(dev, workload):
         xs, ys = leaves()
         xp, objs = autograd.batch_solve_diff(cost.sqdist_packed(xs, ys, pb), pb)
         loss = objs.sum()
-        for i in range(len(shapes)):
-            loss = loss + (w[i] * pb.block(xp, i)).sum()
+        for wi, xi in zip(w, autograd.unpack(xp, pb)):
+            loss = loss + (wi * xi).sum()
         loss.backward()
         return xs
 

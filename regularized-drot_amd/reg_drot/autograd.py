@@ -95,6 +95,35 @@ class _BatchFunction(torch.autograd.Function):
         return grad_cp, None, None
 
 
+class _Unpack(torch.autograd.Function):
+    """Packed buffer -> the per-problem [n_cols, n_rows] views.  Slicing the
+    packed tensor directly would give every view's gradient the full packed
+    size (one fill and one add of the whole buffer per problem); here the
+    backward fills the packed gradient once and copies each problem's block."""
+
+    @staticmethod
+    def forward(ctx, xp, pb):
+        ctx.set_materialize_grads(False)
+        ctx.pb = pb
+        ctx.meta = (xp.shape, xp.dtype, xp.device)
+        return tuple(pb.block(xp, i) for i in range(len(pb.shapes)))
+
+    @staticmethod
+    def backward(ctx, *grads):
+        shape, dtype, device = ctx.meta
+        g = torch.zeros(shape, dtype=dtype, device=device)
+        for i, gi in enumerate(grads):
+            if gi is not None:
+                ctx.pb.block(g, i).copy_(gi)
+        return g, None
+
+
+def unpack(xp, pb):
+    """Per-problem plans (views of the packed xp) whose gradients flow back to
+    xp in one pass; use it instead of pb.block when a loss reads many plans."""
+    return list(_Unpack.apply(xp, pb))
+
+
 def pack_costs(cs, pb) -> torch.Tensor:
     """Differentiable packing of cs into pb's layout (padding rows are zero)."""
     parts = []
@@ -125,5 +154,4 @@ def quadratic_drot_batch_diff(cs, ps, qs, rho, r_weight, max_iters, eps, *, supp
     cp = pack_costs(cs, pb)
     xp, objs = batch_solve_diff(cp, pb, support_tol=support_tol, max_cg_iters=max_cg_iters,
                                 cg_tol=cg_tol, nnz_capacity=nnz_capacity)
-    plans = [pb.block(xp, i) for i in range(len(cs))]
-    return plans, objs
+    return unpack(xp, pb), objs
