@@ -162,8 +162,10 @@ typedef struct drot_batch_problem {
     drot_qr_params params;
 } drot_batch_problem;
 
-/* Fill shapes and a dense packed layout (ld = n_rows rounded up to 4) for a
- * batch; params are left to the caller.  Outputs the packed lengths. */
+/* Fill shapes and a dense packed layout for a batch (ld = n_rows rounded up
+ * to 32 floats, so every column starts on a 128-byte cache line and the
+ * write-through stores of the pass write whole lines); params are left to the
+ * caller.  Outputs the packed lengths. */
 drot_status drot_batch_layout(int n_problems, const int *n_rows, const int *n_cols,
                               drot_batch_problem *out, long long *mat_elems,
                               long long *p_elems, long long *q_elems);

@@ -6,7 +6,7 @@
 cs[i] is an [M_i, N_i] float32 CUDA tensor, ps[i] / qs[i] its marginals
 (the reference binding's layout, pytorch/src/qr_drot_cuda_kernel.cu:89-93);
 r_weight may be one float or one per problem.  The problems are packed into
-the library's batch layout (leading dimension rounded up to 4), solved by
+the library's batch layout (leading dimension rounded up to 32 floats), solved by
 drot_qr_batch_solve_f32, and unpacked; each plan equals
 reg_drot.quadratic_drot on that problem alone.  PackedBatch exposes the packed
 buffers for callers that keep problems resident between solves.

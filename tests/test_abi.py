@@ -137,3 +137,17 @@ def test_tile_cache_rejects_bad_tiles(tmp_path):
     bad.write_text("1000 10 4 16\n")
     with pytest.raises(_abi.DrotError, match="malformed"):
         _abi.qr_tile_cache_load(str(bad))
+
+
+def test_batch_layout_is_dense_and_line_aligned():
+    """drot_batch_layout: columns start on 128-byte lines (ld a multiple of 32
+    floats), problems follow each other without gaps, p and q are packed."""
+    from reg_drot import _abi
+    probs, mat, pe, qe = _abi.batch_layout([5, 64, 100, 33], [3, 2, 7, 1])
+    lds = [probs[i].ld for i in range(4)]
+    assert lds == [32, 64, 128, 64]
+    offs = [probs[i].mat_offset for i in range(4)]
+    assert offs == [0, 32 * 3, 32 * 3 + 64 * 2, 32 * 3 + 64 * 2 + 128 * 7]
+    assert mat == offs[-1] + 64 * 1
+    assert [probs[i].p_offset for i in range(4)] == [0, 5, 69, 169] and pe == 202
+    assert [probs[i].q_offset for i in range(4)] == [0, 3, 5, 12] and qe == 13
