@@ -149,7 +149,7 @@ __global__ __launch_bounds__(kBlock) void batch_finalize_kernel(QrBatchArgs A) {
     F.n_fin = P.n_fin;
     F.blk = f.blk;
     F.with_obj = (A.k % 2) == 0;
-    finalize_block(F, s_part, s_red);
+    finalize_block<8>(F, s_part, s_red);
 }
 
 // core/kernel_qr.hpp:469-482 per converged problem; grid.y = problem.

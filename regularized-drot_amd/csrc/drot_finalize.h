@@ -34,11 +34,14 @@ struct FinJob {
 
 // The block is latency-bound: every global load (the done flag, the partials,
 // p/q, a/b and the objective slice) is issued before the first of them is
-// used, so the block pays one memory round trip before its stores.
+// used, so the block pays one memory round trip before its stores when a row
+// has at most BATCH * kBlock / kFinRows partials.  BATCH (partial loads in
+// flight per thread) does not change the order of the additions.
+template <int BATCH = 8>
 __device__ inline void finalize_block(const FinJob &A, double (*s_part)[kFinRows],
                                       double *s_red) {
     constexpr int kParts = kBlock / kFinRows;
-    constexpr int kBatch = 8;  // partial loads in flight per thread
+    constexpr int kBatch = BATCH;
     const int N = A.n_rows, M = A.n_cols;
     const int b = A.blk;
     const int part = threadIdx.x / kFinRows;

@@ -488,6 +488,7 @@ __global__ __launch_bounds__(kBlock) void gl_wave_kernel(GlPassArgs A) {
 
 // ----------------------------------------------------------------- finalize
 
+template <int BATCH>
 __global__ __launch_bounds__(kBlock) void finalize_kernel(FinalizeArgs A) {
     __shared__ double s_part[kBlock / kFinRows][kFinRows];
     __shared__ double s_red[3 * kWavesPerBlock];
@@ -512,7 +513,7 @@ __global__ __launch_bounds__(kBlock) void finalize_kernel(FinalizeArgs A) {
     F.n_fin = A.rb + A.cb;
     F.blk = blockIdx.x;
     F.with_obj = A.with_obj;
-    finalize_block(F, s_part, s_red);
+    finalize_block<BATCH>(F, s_part, s_red);
 }
 
 // ----------------------------------------------------------------- fix_x
@@ -611,8 +612,18 @@ hipError_t launch_gl_pass(const GlTile &t, int grid, size_t lds, const GlPassArg
     return hipErrorInvalidValue;
 }
 
+// Rows with many partials (GL's column tiles: 250 at 1500x1000) get more loads
+// in flight per thread; few partials keep the short unrolled batch, whose
+// extra predicated loads would cost more than they save.
 hipError_t launch_finalize(int grid, const FinalizeArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(finalize_kernel, dim3(grid), dim3(kBlock), 0, s, a);
+    constexpr int kParts = kBlock / kFinRows;
+    const int cnt = a.n_ct > a.n_rt ? a.n_ct : a.n_rt;
+    if (cnt <= 8 * kParts)
+        hipLaunchKernelGGL(finalize_kernel<8>, dim3(grid), dim3(kBlock), 0, s, a);
+    else if (cnt <= 16 * kParts)
+        hipLaunchKernelGGL(finalize_kernel<16>, dim3(grid), dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL(finalize_kernel<32>, dim3(grid), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 
