@@ -1817,14 +1817,22 @@ drot_status drot_qr_autotune_f32(const float *c, const float *p, const float *q,
     for (const auto &cand : tune_candidates(n_rows, n_cols, v4)) {
         const Plan P = plan_qr_tile(n_rows, n_cols, cand.first, cand.second);
         if (P.total > workspace_bytes) continue;
-        double per = 1e300;
+        // an untimed run first (a kernel instance's first launch loads its
+        // code), then the fastest of three runs at each length: a minimum of
+        // per-repetition differences would pick the repetition whose short
+        // run was slowed, and underestimate the iteration
+        double t_warm = 0.0;
+        if ((s = timed(P, k_lo, &t_warm)) != DROT_OK) return s;
+        double t_lo = 1e300, t_hi = 1e300;
         for (int rep = 0; rep < 3; rep++) {
-            double t_lo = 0.0, t_hi = 0.0;
-            if ((s = timed(P, k_lo, &t_lo)) != DROT_OK) return s;
-            if ((s = timed(P, k_hi, &t_hi)) != DROT_OK) return s;
-            const double d = (t_hi - t_lo) / (double)(k_hi - k_lo);
-            if (d > 0.0) per = std::min(per, d);
+            double a = 0.0, b = 0.0;
+            if ((s = timed(P, k_lo, &a)) != DROT_OK) return s;
+            if ((s = timed(P, k_hi, &b)) != DROT_OK) return s;
+            t_lo = std::min(t_lo, a);
+            t_hi = std::min(t_hi, b);
         }
+        const double d = (t_hi - t_lo) / (double)(k_hi - k_lo);
+        const double per = d > 0.0 ? d : 1e300;
         if (per < best_t) {
             best_t = per;
             arg = cand;
