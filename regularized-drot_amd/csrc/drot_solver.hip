@@ -160,14 +160,44 @@ bool tuned_tile(int n_rows, int n_cols, bool v4, QrTile *t, int *nstep) {
 // ~1024 blocks when the matrix allows (256 CUs, several blocks resident each).
 // The strided-axis partials cost 8 bytes per row per column tile.  A tile set
 // by the autotuner for this shape (or DROT_QR_TILE) takes precedence.
+// Built-in 16-byte tile for shapes without a tuned entry, the rule the tuned
+// MI355X table follows (512^2 .. 10000^2): 16 lanes per column (64-row
+// tiles) and the fewest column steps (at most 8) whose grid fits one resident
+// round of the even pass (kResidentPass blocks: 5 per CU on 256 CUs), grown
+// while tiles hold under kSmallTile elements and the grid keeps kMinGrid
+// blocks; a grid beyond two rounds even at 8 steps (HBM-resident problems)
+// takes 32 lanes per column, 8 steps and plain stores.
+constexpr long long kResidentPass = 1280;
+constexpr long long kMinGrid = 512;
+constexpr int kSmallTile = 4096;
+
+void builtin_tile16(int n_rows, int n_cols, QrTile *t, int *nstep) {
+    constexpr int tr = 64, tc_min = kWavesPerBlock * (kWave / 16);
+    const long long n_rt = ceil_div(n_rows, tr);
+    auto grid = [&](int ns) { return n_rt * ceil_div(n_cols, ns * tc_min); };
+    int ns = 1;
+    while (ns < 8 && grid(ns) > kResidentPass) ns++;
+    if (grid(ns) > 2 * kResidentPass) {
+        *t = QrTile{4, 32, 0, 2};
+        *nstep = 8;
+        return;
+    }
+    while (ns < 8 && tr * tc_min * ns < kSmallTile && grid(ns + 1) >= kMinGrid) ns++;
+    *t = QrTile{4, 16, 2, 2};
+    *nstep = ns;
+}
+
 Plan plan_qr(int n_rows, int n_cols, bool vec4_ok) {
     const bool v4 = vec4_ok && (n_rows % 4 == 0);
     QrTile t;
     int nstep = 0;
     if (!qr_tile_from_env(v4, &t, &nstep) && !tuned_tile(n_rows, n_cols, v4, &t, &nstep)) {
-        if (v4) t = QrTile{4, n_rows >= 1024 ? 32 : 16, 2, 2};
-        else t = QrTile{1, n_rows >= 256 ? 64 : (n_rows >= 32 ? 32 : 16), 2, 2};
-        nstep = 0;
+        if (v4) {
+            builtin_tile16(n_rows, n_cols, &t, &nstep);
+        } else {
+            t = QrTile{1, n_rows >= 256 ? 64 : (n_rows >= 32 ? 32 : 16), 2, 2};
+            nstep = 0;
+        }
     }
     return plan_qr_tile(n_rows, n_cols, t, nstep);
 }
