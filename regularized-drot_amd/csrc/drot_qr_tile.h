@@ -111,11 +111,6 @@ __device__ __forceinline__ void set_issue_priority(int remaining, int nstep) {
     else __builtin_amdgcn_s_setprio(0);
 }
 
-template <int LPC>
-struct QrTileShared {
-    static constexpr int kCpw = kWave / LPC;
-};
-
 template <int VEC, int LPC, bool EVEN, int ST, int U>
 __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, bool leader_block,
                                         double (*s_row)[LPC * VEC], double *s_red) {
