@@ -35,16 +35,20 @@ constexpr int kPolicySc1 = 16;          // cache-policy operand: sc1
 template <int VEC>
 struct Io;
 
+// The 16-byte accesses need only 4-byte alignment (gfx950's multi-dword
+// global and buffer accesses are dword-aligned), so a column may start
+// anywhere: p is read and written through 4-byte-aligned copies.
 template <>
 struct Io<4> {
     static __device__ __forceinline__ void load(const float *p, float (&v)[4]) {
-        const f32x4 t = *reinterpret_cast<const f32x4 *>(p);
+        f32x4 t;
+        __builtin_memcpy(&t, p, sizeof(t));
         v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
     }
     static __device__ __forceinline__ void store(float *p, const float (&v)[4]) {
         f32x4 t;
         t.x = v[0]; t.y = v[1]; t.z = v[2]; t.w = v[3];
-        *reinterpret_cast<f32x4 *>(p) = t;
+        __builtin_memcpy(p, &t, sizeof(t));
     }
     static __device__ __forceinline__ void store_wt(__amdgpu_buffer_rsrc_t r, int byte_off,
                                                     const float (&v)[4]) {
@@ -68,7 +72,10 @@ struct Io<1> {
 };
 
 // Everything one tile needs.  x/c point at the problem's matrix (leading
-// dimension ld >= n_rows, ld % VEC == 0), the vectors at the problem's slices.
+// dimension ld >= n_rows), the vectors at the problem's slices.  When ld is
+// not a multiple of VEC, the last chunk of a column's rows is partial: it is
+// read and written element by element (a 16-byte access would reach into the
+// next column, or past the end of the matrix).
 struct QrTileJob {
     float *x;
     const float *c;
@@ -129,6 +136,9 @@ __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, 
     bool ev_ok[VEC];  // element rows inside the problem (padding rows excluded)
 #pragma unroll
     for (int e = 0; e < VEC; e++) ev_ok[e] = row + e < N;
+    // the whole chunk lies in the column (always when ld % VEC == 0: rows
+    // N..ld-1 are then padding of the same column)
+    const bool row_full = row + VEC <= ld;
     const int tile_col0 = J.ct * J.tc;
     const int col0 = tile_col0 + w * CPW + cs;  // column of step t: col0 + t * kColStride
     const bool explicit_phi = (ctl.k == 0);
@@ -144,9 +154,17 @@ __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, 
             h2[u] = col_ok ? hat2[j] : 0.0f;
             if (col_ok && row_ok) {
                 const size_t off = (size_t)j * (size_t)ld + (size_t)row;
-                Io<VEC>::load(J.x + off, xv[u]);
-                if (EVEN) Io<VEC>::load(J.c + off, cv[u]);
-                else {
+                if (row_full) {
+                    Io<VEC>::load(J.x + off, xv[u]);
+                    if (EVEN) Io<VEC>::load(J.c + off, cv[u]);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < VEC; e++) {
+                        xv[u][e] = ev_ok[e] ? J.x[off + e] : 0.0f;
+                        cv[u][e] = (EVEN && ev_ok[e]) ? J.c[off + e] : 0.0f;
+                    }
+                }
+                if (!EVEN) {
 #pragma unroll
                     for (int e = 0; e < VEC; e++) cv[u][e] = 0.0f;
                 }
@@ -220,13 +238,18 @@ __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, 
                     if (EVEN) oacc += (double)xp * (double)ca[u][e];
                 }
             }
-            if (ok) {
+            if (ok && row_full) {
                 if (ST == 2) {
                     Io<VEC>::store_wt(xrsrc, ((j - tile_col0) * ld + row) * 4, xo);
                 } else {
                     const size_t off = (size_t)j * (size_t)ld + (size_t)row;
                     Io<VEC>::store(J.x + off, xo);
                 }
+            } else if (ok) {
+                const size_t off = (size_t)j * (size_t)ld + (size_t)row;
+#pragma unroll
+                for (int e = 0; e < VEC; e++)
+                    if (ev_ok[e]) J.x[off + e] = xo[e];
             }
             csum = seg_sum<LPC>(csum);
             if (rl == 0 && col_ok) J.pcol[(size_t)J.rt * M + j] = csum;

@@ -188,7 +188,7 @@ void builtin_tile16(int n_rows, int n_cols, QrTile *t, int *nstep) {
 }
 
 Plan plan_qr(int n_rows, int n_cols, bool vec4_ok) {
-    const bool v4 = vec4_ok && (n_rows % 4 == 0);
+    const bool v4 = vec4_ok;
     QrTile t;
     int nstep = 0;
     if (!qr_tile_from_env(v4, &t, &nstep) && !tuned_tile(n_rows, n_cols, v4, &t, &nstep)) {
@@ -1755,8 +1755,6 @@ drot_status drot_qr_tile_cache_set(int n_rows, int n_cols, const drot_qr_tile *t
     const QrTile t{tile->vec, tile->lanes_per_col, tile->store_policy, tile->unroll};
     if (!qr_tile_supported(t) || tile->col_steps < 1 || tile->col_steps > 64)
         return fail(DROT_ERR_INVALID_ARGUMENT, "unsupported tile");
-    if (t.vec == 4 && n_rows % 4 != 0)
-        return fail(DROT_ERR_INVALID_ARGUMENT, "vec 4 needs n_rows % 4 == 0");
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned[{n_rows, n_cols}] = TunedTile{t, tile->col_steps, 0.0};
     return DROT_OK;
@@ -1811,7 +1809,7 @@ drot_status drot_qr_autotune_workspace_size(int n_rows, int n_cols, size_t *byte
     if (!bytes || n_rows <= 0 || n_cols <= 0)
         return fail(DROT_ERR_INVALID_ARGUMENT, "bad arguments to drot_qr_autotune_workspace_size");
     size_t mx = 0;
-    for (const auto &cand : tune_candidates(n_rows, n_cols, n_rows % 4 == 0))
+    for (const auto &cand : tune_candidates(n_rows, n_cols, true))
         mx = std::max(mx, plan_qr_tile(n_rows, n_cols, cand.first, cand.second).total);
     size_t ws = 0;
     drot_qr_workspace_size(n_rows, n_cols, &ws);
@@ -1833,7 +1831,7 @@ drot_status drot_qr_autotune_f32(const float *c, const float *p, const float *q,
     const int k_hi = std::max(std::max(8, iters), std::min(4000, (int)(4000.0 / est_us))) & ~1;
     const int k_lo = std::max(2, (k_hi / 5) & ~1);
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    const bool v4 = aligned16(c) && aligned16(x) && n_rows % 4 == 0;
+    const bool v4 = aligned16(c) && aligned16(x);
     double best_t = 1e300;
     std::pair<QrTile, int> arg{};
     bool any = false;

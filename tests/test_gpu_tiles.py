@@ -47,12 +47,13 @@ def _rel(a, b):
     return float(np.abs(a.astype(np.float64) - b).max()) / float(np.abs(b).max())
 
 
+@pytest.mark.parametrize("n_rows", [260, 257])
 @pytest.mark.parametrize("tile", TILES, ids=lambda t: "v%d_l%d_n%d_st%d_u%d" % t)
-def test_tile_trajectory_matches_oracle(device, tile):
+def test_tile_trajectory_matches_oracle(device, tile, n_rows):
     from reg_drot import _abi
-    vec = tile[0]
-    # 16-byte tiles need n_rows % 4 == 0; both shapes leave partial tiles
-    n_rows, n_cols = (260, 203) if vec == 4 else (257, 203)
+    # both shapes leave partial row and column tiles; with 257 rows the
+    # columns are not 16-byte aligned and end in a partial 4-row chunk
+    n_cols = 203
     c = problems.reference_test_cost(n_rows, n_cols, seed=17)
     p, q = problems.uniform_marginals(n_rows, n_cols)
     prm = _abi.qr_params_from_rho(2.0, 1e-2, n_rows, n_cols, 9, 0.0)
