@@ -256,8 +256,9 @@ typedef struct drot_qr_tile {
 /* The tile a solve of this shape will use (tuned, else the built-in choice). */
 drot_status drot_qr_tile_for(int n_rows, int n_cols, int vec4_ok, drot_qr_tile *out);
 
-/* Autotuner: times every candidate tile on this problem (k/5..k iterations
- * from the warm start, k = max(iters, about 4 ms of work); x is scratch)
+/* Autotuner: times every candidate tile on this problem (iterations 64 + k/5
+ * .. 64 + k from the warm start, k = max(iters, about 4 ms of work), so the
+ * plan's zero segments have formed; x is scratch)
  * and records the fastest for (n_rows, n_cols) in the process-wide tile cache,
  * which later solves of that shape use.  The tuned tile can change
  * drot_qr_workspace_size: query it after tuning.  Workspace:

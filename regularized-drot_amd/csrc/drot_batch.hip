@@ -119,8 +119,10 @@ __global__ __launch_bounds__(kBlock) void batch_pass_kernel(QrBatchArgs A) {
     J.tc = P.nstep * kWavesPerBlock * CPW;
     J.step = P.step;
     J.kappa = P.kappa;
+    J.zin = nullptr;
+    J.zout = nullptr;
     const IterCtl ctl = batch_ctl(A, P, prob);
-    qr_tile<4, LPC, EVEN, 2, 2>(J, ctl, bt.rt == 0 && bt.ct == 0, s_row, s_red);
+    qr_tile<4, LPC, EVEN, 2, 2, false>(J, ctl, bt.rt == 0 && bt.ct == 0, s_row, s_red);
 }
 
 __global__ __launch_bounds__(kBlock) void batch_finalize_kernel(QrBatchArgs A) {

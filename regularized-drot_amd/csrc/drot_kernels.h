@@ -23,6 +23,10 @@ struct QrPassArgs {
     int nstep;                 // column steps per wave; tile columns = nstep * 4 * (64 / lpc)
     int prio;                  // set by launch_qr_pass: 1 = issue priority by remaining steps
     float step, kappa;
+    // zero-segment flags ([n_rt][n_cols] bytes; null: off): zin from the
+    // previous pass, zout written by this one (see drot_qr_tile.h)
+    const unsigned char *zin;
+    unsigned char *zout;
     IterCtl ctl;
 };
 
@@ -64,6 +68,8 @@ struct InitArgs {
     int n_rows, n_cols;
     int warm;
     float v_phi1, v_phi2, v_a, v_b, v_alpha;
+    unsigned char *zf;  // QR zero-segment flags, cleared here (null: none)
+    size_t zf_bytes;
 };
 
 struct QrTile {
@@ -90,6 +96,11 @@ hipError_t launch_gl_pass(const GlTile &t, int grid, size_t lds_bytes,
 hipError_t launch_finalize(int grid, const FinalizeArgs &a, hipStream_t s);
 hipError_t launch_fix_x(float *x, const float *c, float step, size_t total,
                         const SolverState *st, hipStream_t s);
+// Writes the stored value into the segments the last executed QR pass left
+// flagged (zf: both flag buffers, zf_bytes each; tr rows per segment).
+hipError_t launch_qr_unflag(float *x, const float *c, float step, const unsigned char *zf,
+                            size_t zf_bytes, int n_rows, int n_cols, int tr,
+                            const SolverState *st, hipStream_t s);
 hipError_t launch_export_phi(const float *ahat, const float *bhat, const float *phi1,
                              const float *phi2, const SolverState *st, int n_rows,
                              int n_cols, float *out_phi1, float *out_phi2,

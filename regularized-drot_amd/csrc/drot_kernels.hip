@@ -87,6 +87,9 @@ __global__ __launch_bounds__(kBlock) void init_kernel(InitArgs A) {
         A.b[j] = A.warm ? A.v_b : 0.0f;
         A.bhat[j] = 0.0f;
     }
+    // no segment starts out flagged: the first pass reads all of X
+    if (A.zf)
+        for (size_t i = tid; i < A.zf_bytes; i += nthr) A.zf[i] = 0;
     if (tid == 0) {
         SolverState s;
         s.done = 0;
@@ -543,6 +546,30 @@ __global__ __launch_bounds__(kBlock) void fix_x_kernel(float *x, const float *c,
     for (; e < total; e += nthr) x[e] = fmaxf(__builtin_fmaf(step, c[e], x[e]), 0.0f);
 }
 
+// After the loop: the segments the last executed pass (k_run - 1) flagged as
+// all-zero were not stored, so write what that pass would have stored there:
+// X' - sC with X' = +0 (fmaf(-s, c, 0), the pass's own expression) after an
+// even pass, X' = +0 after an odd one.  Segment (rt, j) covers rows
+// [rt*tr, rt*tr + tr) of column j; its flag is zf[(k & 1) * zf_bytes + rt*n_cols + j].
+__global__ __launch_bounds__(kBlock) void qr_unflag_kernel(float *x, const float *c, float step,
+                                                          const unsigned char *zf,
+                                                          size_t zf_bytes, int n_rows,
+                                                          int n_cols, int tr,
+                                                          const SolverState *st) {
+    const int k_last = st->k_run - 1;
+    if (k_last < 0) return;
+    const unsigned char *z = zf + (size_t)(k_last & 1) * zf_bytes;
+    const bool even = (k_last & 1) == 0;
+    const size_t total = (size_t)n_rows * (size_t)n_cols;
+    const size_t nthr = (size_t)gridDim.x * blockDim.x;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += nthr) {
+        const int j = (int)(e / (size_t)n_rows);
+        const int i = (int)(e - (size_t)j * (size_t)n_rows);
+        if (z[(size_t)(i / tr) * (size_t)n_cols + j])
+            x[e] = even ? __builtin_fmaf(-step, c[e], 0.0f) : 0.0f;
+    }
+}
+
 // Materialise phi1/phi2 of the current state (for state export / resume).
 __global__ __launch_bounds__(kBlock) void export_phi_kernel(const float *ahat, const float *bhat,
                                                            const float *phi1, const float *phi2,
@@ -631,6 +658,15 @@ hipError_t launch_fix_x(float *x, const float *c, float step, size_t total,
                         const SolverState *st, hipStream_t s) {
     const int grid = grid_for(total / 4 + 1, kBlock * 4, 8192);
     hipLaunchKernelGGL(fix_x_kernel, dim3(grid), dim3(kBlock), 0, s, x, c, step, total, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_qr_unflag(float *x, const float *c, float step, const unsigned char *zf,
+                            size_t zf_bytes, int n_rows, int n_cols, int tr,
+                            const SolverState *st, hipStream_t s) {
+    const int grid = grid_for((size_t)n_rows * (size_t)n_cols, kBlock, 8192);
+    hipLaunchKernelGGL(qr_unflag_kernel, dim3(grid), dim3(kBlock), 0, s, x, c, step, zf,
+                       zf_bytes, n_rows, n_cols, tr, st);
     return hipGetLastError();
 }
 

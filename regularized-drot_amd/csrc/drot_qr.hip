@@ -15,7 +15,8 @@
 namespace drot {
 
 // ST: store policy of X (see Io); ST == 2 needs tc * n_rows * 4 < 2^31.
-template <int VEC, int LPC, bool EVEN, int ST, int U>
+// ZS: zero-segment flags (A.zin / A.zout) in use.
+template <int VEC, int LPC, bool EVEN, int ST, int U, bool ZS>
 __global__ __launch_bounds__(kBlock) void qr_pass_kernel(QrPassArgs A) {
     constexpr int CPW = kWave / LPC;
     constexpr int TR = LPC * VEC;
@@ -53,7 +54,9 @@ __global__ __launch_bounds__(kBlock) void qr_pass_kernel(QrPassArgs A) {
     J.tc = A.nstep * kWavesPerBlock * CPW;
     J.step = A.step;
     J.kappa = A.kappa;
-    qr_tile<VEC, LPC, EVEN, ST, U>(J, A.ctl, blockIdx.x == 0, s_row, s_red);
+    J.zin = A.zin;
+    J.zout = A.zout;
+    qr_tile<VEC, LPC, EVEN, ST, U, ZS>(J, A.ctl, blockIdx.x == 0, s_row, s_red);
     if (A.ctl.tspan) record_span(A.ctl.tspan, t0);
 }
 
@@ -86,27 +89,35 @@ static int grid_capacity(const void *k, std::atomic<int> *cache) {
     return cap;
 }
 
-template <bool EVEN, int VEC, int LPC, int ST, int U>
+template <bool EVEN, int VEC, int LPC, int ST, int U, bool ZS>
 static int pass_capacity() {
     static std::atomic<int> cache[kMaxDevices];  // zero-initialised (static storage)
-    return grid_capacity(reinterpret_cast<const void *>(qr_pass_kernel<VEC, LPC, EVEN, ST, U>),
-                         cache);
+    return grid_capacity(
+        reinterpret_cast<const void *>(qr_pass_kernel<VEC, LPC, EVEN, ST, U, ZS>), cache);
 }
 
-template <int VEC, int LPC, int ST, int U>
-static void launch_pass_t(bool even, int grid, const QrPassArgs &a, hipStream_t s,
+template <int VEC, int LPC, int ST, int U, bool ZS>
+static void launch_pass_z(bool even, int grid, const QrPassArgs &a, hipStream_t s,
                           hipEvent_t e0, hipEvent_t e1) {
-    auto k = even ? qr_pass_kernel<VEC, LPC, true, ST, U> : qr_pass_kernel<VEC, LPC, false, ST, U>;
+    auto k = even ? qr_pass_kernel<VEC, LPC, true, ST, U, ZS>
+                  : qr_pass_kernel<VEC, LPC, false, ST, U, ZS>;
     // ranking waves by remaining work pays only when the whole grid is resident
     // (one round of blocks); with later rounds it costs a few percent
-    const int cap = even ? pass_capacity<true, VEC, LPC, ST, U>()
-                         : pass_capacity<false, VEC, LPC, ST, U>();
+    const int cap = even ? pass_capacity<true, VEC, LPC, ST, U, ZS>()
+                         : pass_capacity<false, VEC, LPC, ST, U, ZS>();
     QrPassArgs b = a;
     b.prio = grid <= cap ? 1 : 0;
     if (e0 && e1)  // timing: events bracket the kernel's own execution
         hipExtLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, b);
     else
         hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, b);
+}
+
+template <int VEC, int LPC, int ST, int U>
+static void launch_pass_t(bool even, int grid, const QrPassArgs &a, hipStream_t s,
+                          hipEvent_t e0, hipEvent_t e1) {
+    if (a.zin && a.zout) launch_pass_z<VEC, LPC, ST, U, true>(even, grid, a, s, e0, e1);
+    else launch_pass_z<VEC, LPC, ST, U, false>(even, grid, a, s, e0, e1);
 }
 
 template <int VEC, int LPC>

@@ -14,6 +14,18 @@
 // Software pipelined: the first batch of X/C loads (and the row potentials)
 // is issued before the prologue's reductions, and every batch's loads are in
 // flight while the previous batch is computed and stored.
+//
+// Zero segments.  The plan is sparse long before convergence (at 3000^2 about
+// 95% of the 64-row column segments of X' are all zero after a tenth of the
+// iterations).  A segment is the TR rows of one column inside a tile.  When
+// every X' of a segment is +0 (bit pattern 0), the pass stores nothing there
+// and sets the segment's flag in zout; what it would have stored is known
+// without the store: +0 after an odd pass, fmaf(-s, c, +0) after an even one.
+// The next pass reads its flags (zin) first: on a flagged segment the even
+// pass skips the X load (X = +0) and the odd pass loads C in place of X and
+// forms fmaf(-s, c, +0).  Every element sees the same operands as without the
+// flags, so results keep their bits, and a settled segment moves 4 bytes per
+// element (C) instead of 12 (even) or 8 (odd).
 #pragma once
 
 #include "drot_kernels.h"
@@ -87,6 +99,10 @@ struct QrTileJob {
     int rt, ct, tc, nstep;
     int prio;       // 1: waves raise their issue priority while many column steps remain
     float step, kappa;
+    // zero-segment flags, [n_rt][n_cols] bytes, both null or both set
+    // (needs nstep <= LPC: lane rl of a column group loads the flag of step rl)
+    const unsigned char *zin;
+    unsigned char *zout;
 };
 
 // y / kappa, correctly rounded, from the correctly rounded reciprocal
@@ -118,7 +134,19 @@ __device__ __forceinline__ void set_issue_priority(int remaining, int nstep) {
     else __builtin_amdgcn_s_setprio(0);
 }
 
-template <int VEC, int LPC, bool EVEN, int ST, int U>
+// The LPC bits of ballot b that belong to column group cs (lanes cs*LPC ...).
+template <int LPC>
+__device__ __forceinline__ unsigned long long group_bits(unsigned long long b, int cs) {
+    if constexpr (LPC == 64) {
+        return b;
+    } else {
+        return (b >> (cs * LPC)) & ((1ull << LPC) - 1ull);
+    }
+}
+
+// ZS: zero-segment flags in use (J.zin and J.zout set); without them the
+// loads are the plain ones below.
+template <int VEC, int LPC, bool EVEN, int ST, int U, bool ZS>
 __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, bool leader_block,
                                         double (*s_row)[LPC * VEC], double *s_red) {
     constexpr int CPW = kWave / LPC;  // columns per wave step
@@ -144,34 +172,75 @@ __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, 
     const bool explicit_phi = (ctl.k == 0);
     const float *hat2 = explicit_phi ? J.phi2 : J.bhat;
 
+    // flags of this column group's segments from the previous pass: bit t is
+    // column step t (loaded one per lane, gathered by a ballot)
+    constexpr bool zon = ZS;
+    unsigned zmask = 0;
+    if (zon) {
+        const int jz = col0 + rl * kColStride;
+        const bool zb = rl < nstep && jz < M && J.zin[(size_t)J.rt * M + jz] != 0;
+        const unsigned long long bz = __builtin_amdgcn_ballot_w64(zb);
+        zmask = (unsigned)group_bits<LPC>(bz, cs);
+    }
+
+    // the matrix pointers as locals: a select between two members of J
+    // would be a load through a selected address, which keeps J in scratch
+    const float *const xptr = J.x;
+    const float *const cptr = J.c;
     float xa[U][VEC], ca[U][VEC], h2a[U];
-    auto load_batch = [&](int t0, float (&xv)[U][VEC], float (&cv)[U][VEC], float (&h2)[U]) {
+    // always inlined: the batches are register arrays (a call would put them in scratch)
+    auto load_batch = [&](int t0, float (&xv)[U][VEC], float (&cv)[U][VEC],
+                          float (&h2)[U]) __attribute__((always_inline)) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int t = t0 + u;
             const int j = col0 + t * kColStride;
             const bool col_ok = (t < nstep) && (j < M);
             h2[u] = col_ok ? hat2[j] : 0.0f;
-            if (col_ok && row_ok) {
-                const size_t off = (size_t)j * (size_t)ld + (size_t)row;
-                if (row_full) {
-                    Io<VEC>::load(J.x + off, xv[u]);
-                    if (EVEN) Io<VEC>::load(J.c + off, cv[u]);
+            if constexpr (!ZS) {
+                if (col_ok && row_ok) {
+                    const size_t off = (size_t)j * (size_t)ld + (size_t)row;
+                    if (row_full) {
+                        Io<VEC>::load(J.x + off, xv[u]);
+                        if (EVEN) Io<VEC>::load(J.c + off, cv[u]);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < VEC; e++) {
+                            xv[u][e] = ev_ok[e] ? J.x[off + e] : 0.0f;
+                            cv[u][e] = (EVEN && ev_ok[e]) ? J.c[off + e] : 0.0f;
+                        }
+                    }
+                    if (!EVEN) {
+#pragma unroll
+                        for (int e = 0; e < VEC; e++) cv[u][e] = 0.0f;
+                    }
                 } else {
 #pragma unroll
-                    for (int e = 0; e < VEC; e++) {
-                        xv[u][e] = ev_ok[e] ? J.x[off + e] : 0.0f;
-                        cv[u][e] = (EVEN && ev_ok[e]) ? J.c[off + e] : 0.0f;
-                    }
+                    for (int e = 0; e < VEC; e++) { xv[u][e] = 0.0f; cv[u][e] = 0.0f; }
                 }
-                if (!EVEN) {
-#pragma unroll
-                    for (int e = 0; e < VEC; e++) cv[u][e] = 0.0f;
-                }
-            } else {
-#pragma unroll
-                for (int e = 0; e < VEC; e++) { xv[u][e] = 0.0f; cv[u][e] = 0.0f; }
+                continue;
             }
+            // flagged segment: even reads no X (+0), odd reads C in X's place
+            const bool z = (zmask >> t) & 1u;
+            const bool live = col_ok && row_ok;
+            const size_t off = (size_t)j * (size_t)ld + (size_t)row;
+            const float *xs = (!EVEN && z) ? cptr : xptr;
+            float xt[VEC], ct[VEC];
+#pragma unroll
+            for (int e = 0; e < VEC; e++) { xt[e] = 0.0f; ct[e] = 0.0f; }
+            if (live && row_full) {
+                // C first: its load does not wait for the flags
+                if (EVEN) Io<VEC>::load(J.c + off, ct);
+                if (!(EVEN && z)) Io<VEC>::load(xs + off, xt);
+            } else if (live) {
+#pragma unroll
+                for (int e = 0; e < VEC; e++) {
+                    if (EVEN && ev_ok[e]) ct[e] = J.c[off + e];
+                    if (ev_ok[e] && !(EVEN && z)) xt[e] = xs[off + e];
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < VEC; e++) { xv[u][e] = xt[e]; cv[u][e] = ct[e]; }
         }
     };
     load_batch(0, xa, ca, h2a);
@@ -210,6 +279,11 @@ __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, 
             const bool col_ok = j < M;
             const bool ok = col_ok && row_ok;
             const float f2 = explicit_phi ? h2a[u] : (h2a[u] + gamma) / fN;
+            if (!EVEN && ((zmask >> t) & 1u)) {
+                // what the even pass would have stored: X' - sC with X' = +0
+#pragma unroll
+                for (int e = 0; e < VEC; e++) xa[u][e] = __builtin_fmaf(-s, xa[u][e], 0.0f);
+            }
             float xo[VEC], yv[VEC], xq[VEC];
             double csum = 0.0;
             bool slow = false;
@@ -228,6 +302,7 @@ __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, 
                 for (int e = 0; e < VEC; e++)
                     if (div_kappa_slow(yv[e], xq[e])) xq[e] = yv[e] / kappa;
             }
+            bool nz = false;
 #pragma unroll
             for (int e = 0; e < VEC; e++) {
                 const float xp = xq[e];
@@ -236,9 +311,19 @@ __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, 
                     racc[e] += (double)xp;
                     csum += (double)xp;
                     if (EVEN) oacc += (double)xp * (double)ca[u][e];
+                    nz |= __float_as_uint(xp) != 0u;
                 }
             }
-            if (ok && row_full) {
+            // all of the segment's X' are +0: flag it and store nothing
+            bool zseg = false;
+            if (zon) {
+                const unsigned long long bn = __builtin_amdgcn_ballot_w64(nz);
+                zseg = group_bits<LPC>(bn, cs) == 0ull;
+                if (rl == 0 && col_ok) J.zout[(size_t)J.rt * M + j] = zseg ? 1 : 0;
+            }
+            if (zseg) {
+                // nothing to store
+            } else if (ok && row_full) {
                 if (ST == 2) {
                     Io<VEC>::store_wt(xrsrc, ((j - tile_col0) * ld + row) * 4, xo);
                 } else {

@@ -66,6 +66,9 @@ struct Plan {
     size_t off_state = 0, off_phi1 = 0, off_phi2 = 0, off_a = 0, off_b = 0;
     size_t off_ahat = 0, off_bhat = 0, off_prow = 0, off_pcol = 0, off_pobj = 0;
     size_t off_pfin = 0, off_bounds = 0, off_rowgroup = 0, total = 0;
+    // QR zero-segment flags: two buffers of zf_bytes ([n_rt][n_cols]) at
+    // off_zf, or none (zf_bytes == 0)
+    size_t off_zf = 0, zf_bytes = 0;
 };
 
 // Optional override of the QR tile, "vec,lpc,nstep[,nt[,u]]" (tuning sweeps only).
@@ -104,8 +107,14 @@ void layout(Plan &P) {
     P.off_pfin = take((size_t)(P.rb + P.cb) * 24);
     P.off_bounds = take((size_t)(P.n_groups + 1) * 4);
     P.off_rowgroup = take(P.kind == kGL ? N * 4 : 0);
+    P.off_zf = take(2 * P.zf_bytes);
     P.total = off;
 }
+
+// Matrix size (elements) from which QR solves use zero-segment flags.  Time
+// to 1e-4 with / without them on MI355X: 1000^2 to 2048^2 1.01-1.03x,
+// 2500^2 0.99x, 3000^2 0.94x, 10000^2 0.69x per iteration.
+constexpr long long kZeroSkipMinElems = 6000000;
 
 // Plan of a QR solve with a given pass tile; nstep <= 0 picks the column steps.
 Plan plan_qr_tile(int n_rows, int n_cols, QrTile t, int nstep) {
@@ -131,6 +140,16 @@ Plan plan_qr_tile(int n_rows, int n_cols, QrTile t, int nstep) {
     P.n_obj = P.n_rt * P.n_ct;
     P.rb = ceil_div(n_rows, kFinRows);
     P.cb = ceil_div(n_cols, kFinRows);
+    // zero-segment flags (drot_qr_tile.h) from kZeroSkipMinElems elements up:
+    // they save HBM traffic, while smaller matrices are bound by latency,
+    // which the flag load at the start of a tile adds to.  A column group
+    // gathers its steps' flags one per lane, so nstep <= lpc.
+    // DROT_QR_ZSKIP=1 / 0 turns them on / off for every size.
+    const char *zs = getenv("DROT_QR_ZSKIP");
+    const bool big = (long long)n_rows * n_cols >= kZeroSkipMinElems;
+    const bool want = (zs && zs[0] == '1') || (big && !(zs && zs[0] == '0'));
+    const bool zskip = nstep <= P.qt.lpc && want;
+    P.zf_bytes = zskip ? (size_t)P.n_rt * (size_t)n_cols : 0;
     layout(P);
     return P;
 }
@@ -350,6 +369,7 @@ struct DevPtrs {
     double *prow, *pcol, *pobj, *pfin;
     int *bounds;
     int *row_group;
+    unsigned char *zf;  // null when the plan has no zero-segment flags
 };
 
 DevPtrs carve(const Plan &P, void *ws) {
@@ -368,6 +388,7 @@ DevPtrs carve(const Plan &P, void *ws) {
     d.pfin = reinterpret_cast<double *>(base + P.off_pfin);
     d.bounds = reinterpret_cast<int *>(base + P.off_bounds);
     d.row_group = reinterpret_cast<int *>(base + P.off_rowgroup);
+    d.zf = P.zf_bytes ? reinterpret_cast<unsigned char *>(base + P.off_zf) : nullptr;
     return d;
 }
 
@@ -425,6 +446,8 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
     ia.n_rows = P.n_rows;
     ia.n_cols = P.n_cols;
     ia.warm = in.warm;
+    ia.zf = d.zf;
+    ia.zf_bytes = 2 * P.zf_bytes;
     warm_constants(P.n_rows, P.n_cols, in.step, &ia);
     DROT_HIP_CHECK(launch_init(ia, stream));
 
@@ -591,6 +614,9 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
             if (P.kind == kQR) {
                 qa.ctl.k = k;
                 qa.ctl.tspan = span_ptr(k);
+                // pass k reads the flags pass k-1 wrote and writes the others
+                qa.zin = d.zf ? d.zf + (size_t)((k + 1) & 1) * P.zf_bytes : nullptr;
+                qa.zout = d.zf ? d.zf + (size_t)(k & 1) * P.zf_bytes : nullptr;
                 DROT_HIP_CHECK(launch_qr_pass(P.qt, (k % 2) == 0, P.grid_pass, qa, stream, e0, e1));
             } else {
                 ga.ctl.k = k;
@@ -620,6 +646,9 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
         slot = other;
     }
     if (P.kind == kQR) {
+        if (d.zf)
+            DROT_HIP_CHECK(launch_qr_unflag(in.x, in.c, in.step, d.zf, P.zf_bytes, P.n_rows,
+                                            P.n_cols, P.qt.lpc * P.qt.vec, d.st, stream));
         DROT_HIP_CHECK(launch_fix_x(in.x, in.c, in.step,
                                     (size_t)P.n_rows * (size_t)P.n_cols, d.st, stream));
     }
@@ -1825,11 +1854,16 @@ drot_status drot_qr_autotune_f32(const float *c, const float *p, const float *q,
     if (s != DROT_OK) return s;
     if (!prm || !workspace) return fail(DROT_ERR_INVALID_ARGUMENT, "null pointer argument");
     if (!(prm->scale > 0.0f)) return fail(DROT_ERR_INVALID_ARGUMENT, "scale must be > 0");
-    // iterations k_lo..k_hi with k_hi sized to a few milliseconds of work, so
-    // the difference of the two run times measures the iterations alone
+    // iterations k_lo..k_hi with k_hi - k_lo sized to a few milliseconds of
+    // work, so the difference of the two run times measures those iterations
+    // alone.  Both runs share a settling prefix of kSettle iterations: the
+    // plan's zero segments (whose X traffic the pass skips) form within the
+    // first few dozen iterations, and the rest of a solve runs in that regime.
+    constexpr int kSettle = 64;
     const double est_us = 6.0 + (double)n_rows * (double)n_cols * 10.0 / 5.0e6;
-    const int k_hi = std::max(std::max(8, iters), std::min(4000, (int)(4000.0 / est_us))) & ~1;
-    const int k_lo = std::max(2, (k_hi / 5) & ~1);
+    const int k_len = std::max(std::max(8, iters), std::min(4000, (int)(4000.0 / est_us))) & ~1;
+    const int k_lo = kSettle + std::max(2, (k_len / 5) & ~1);
+    const int k_hi = kSettle + k_len;
     const hipStream_t st = static_cast<hipStream_t>(stream);
     const bool v4 = aligned16(c) && aligned16(x);
     double best_t = 1e300;

@@ -1,0 +1,86 @@
+"""Zero-segment skipping in the QR pass keeps every bit.
+
+A pass stores nothing for a column segment whose X' are all +0 and flags it;
+the next pass forms the stored value (+0 after an odd pass, fmaf(-s, c, +0)
+after an even one) instead of loading it, and after the loop the flagged
+segments of the last pass are written out.  DROT_QR_ZSKIP=0 turns the flags
+off, so the same solve with and without them must give identical bits:
+  - stopped by max_iters after an odd and after an even pass (the written-out
+    segments then hold X' and X' - sC, the reference's stored iterate);
+  - converged (fix_x runs over the written-out segments);
+  - ragged row counts (partial last chunk) and every tile family, including
+    16 column steps (the most flags one column group gathers).
+The plans here are sparse from the first tens of iterations, so most segments
+take the flagged path.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from reg_drot import problems
+
+pytestmark = pytest.mark.gpu
+
+
+def _solve(c, p, q, prm, device, zskip, tile=None):
+    from test_gpu_qr import gpu_qr
+    old = {k: os.environ.get(k) for k in ("DROT_QR_ZSKIP", "DROT_QR_TILE")}
+    try:
+        os.environ["DROT_QR_ZSKIP"] = "1" if zskip else "0"
+        if tile:
+            os.environ["DROT_QR_TILE"] = tile
+        else:
+            os.environ.pop("DROT_QR_TILE", None)
+        return gpu_qr(c, p, q, prm, device)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def _same(a, b):
+    (xa, ra, p1a, p2a, al_a), (xb, rb, p1b, p2b, al_b) = a, b
+    assert ra.n_iter == rb.n_iter and ra.converged == rb.converged
+    assert ra.objective_f64 == rb.objective_f64
+    assert np.array_equal(xa.view(np.uint32), xb.view(np.uint32))
+    assert np.array_equal(p1a.view(np.uint32), p1b.view(np.uint32))
+    assert np.array_equal(p2a.view(np.uint32), p2b.view(np.uint32))
+    assert al_a == al_b
+
+
+@pytest.mark.parametrize("n_rows,n_cols,iters,eps", [
+    (3000, 3000, 201, 0.0), (3000, 3000, 200, 0.0), (1000, 1000, 20000, 1e-4),
+    (1001, 777, 301, 0.0), (1001, 777, 20000, 1e-4), (255, 2048, 20000, 1e-4)])
+def test_flags_keep_bits(device, n_rows, n_cols, iters, eps):
+    from reg_drot import _abi
+    c = problems.cloud_cost(n_rows, n_cols, dim=16, seed=5)
+    p, q = problems.uniform_marginals(n_rows, n_cols)
+    prm = _abi.qr_params_from_rho(2.0, 1e-3, n_rows, n_cols, iters, eps)
+    print(f"[zero segments] {n_rows}x{n_cols} iters={iters} eps={eps}", flush=True)
+    on = _solve(c, p, q, prm, device, True)
+    off = _solve(c, p, q, prm, device, False)
+    if eps > 0:
+        assert on[1].converged == 1
+    _same(on, off)
+    # the plan is sparse: most of it is zero once converged (up to the residue,
+    # at most half an ulp of sC, that fix_x leaves where X' = 0)
+    if eps > 0:
+        x = on[0]
+        assert (x <= 1e-6 * x.max()).mean() > 0.5
+
+
+@pytest.mark.parametrize("tile", ["4,16,7,2,2", "4,32,3", "4,64,2,2,4", "1,16,5",
+                                  "1,32,16,0,2", "4,16,16,2,4", "1,64,1"])
+def test_flags_keep_bits_every_tile(device, tile):
+    from reg_drot import _abi
+    n_rows, n_cols = 1003, 641
+    c = problems.cloud_cost(n_rows, n_cols, dim=16, seed=6)
+    p, q = problems.uniform_marginals(n_rows, n_cols)
+    for iters, eps in ((157, 0.0), (20000, 1e-4)):
+        prm = _abi.qr_params_from_rho(2.0, 1e-3, n_rows, n_cols, iters, eps)
+        print(f"[zero segments] tile {tile} iters={iters}", flush=True)
+        _same(_solve(c, p, q, prm, device, True, tile),
+              _solve(c, p, q, prm, device, False, tile))
