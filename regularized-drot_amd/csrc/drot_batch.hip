@@ -63,6 +63,13 @@ __global__ __launch_bounds__(kBlock) void batch_init_kernel(QrBatchArgs A) {
         A.b[P.col_off + j] = P.warm ? P.v_b : 0.0f;
         A.bhat[P.col_off + j] = 0.0f;
     }
+    if (A.zf) {  // no segment starts out flagged
+        const size_t nz = (size_t)P.n_rt * (size_t)P.n_cols;
+        for (size_t i = tid; i < nz; i += nthr) {
+            A.zf[P.zf_off + i] = 0;
+            A.zf[A.zf_len + P.zf_off + i] = 0;
+        }
+    }
     if (tid == 0) {
         SolverState s;
         s.done = 0;
@@ -86,7 +93,7 @@ __global__ __launch_bounds__(kBlock) void batch_init_kernel(QrBatchArgs A) {
 // One pass over every listed tile; the list holds the tiles of the active
 // problems in row-tile-major order per problem, and blocks take contiguous
 // runs of it per XCD (see qr_pass_kernel).
-template <bool EVEN>
+template <bool EVEN, bool ZS>
 __global__ __launch_bounds__(kBlock) void batch_pass_kernel(QrBatchArgs A) {
     constexpr int LPC = kBatchLpc;
     constexpr int CPW = kWave / LPC;
@@ -119,10 +126,10 @@ __global__ __launch_bounds__(kBlock) void batch_pass_kernel(QrBatchArgs A) {
     J.tc = P.nstep * kWavesPerBlock * CPW;
     J.step = P.step;
     J.kappa = P.kappa;
-    J.zin = nullptr;
-    J.zout = nullptr;
+    J.zin = ZS ? A.zf + (size_t)((A.k + 1) & 1) * A.zf_len + P.zf_off : nullptr;
+    J.zout = ZS ? A.zf + (size_t)(A.k & 1) * A.zf_len + P.zf_off : nullptr;
     const IterCtl ctl = batch_ctl(A, P, prob);
-    qr_tile<4, LPC, EVEN, 2, 2, false>(J, ctl, bt.rt == 0 && bt.ct == 0, s_row, s_red);
+    qr_tile<4, LPC, EVEN, 2, 2, ZS>(J, ctl, bt.rt == 0 && bt.ct == 0, s_row, s_red);
 }
 
 __global__ __launch_bounds__(kBlock) void batch_finalize_kernel(QrBatchArgs A) {
@@ -154,17 +161,33 @@ __global__ __launch_bounds__(kBlock) void batch_finalize_kernel(QrBatchArgs A) {
     finalize_block<8>(F, s_part, s_red);
 }
 
-// core/kernel_qr.hpp:469-482 per converged problem; grid.y = problem.
+// Per problem, after the loop; grid.y = problem.  First the segments that the
+// problem's last executed pass (k_run - 1) left flagged get the value that pass
+// would have stored (fmaf(-s, c, +0) after an even pass, +0 after an odd one;
+// see drot_qr_tile.h), then core/kernel_qr.hpp:469-482 if it converged.
 __global__ __launch_bounds__(kBlock) void batch_fix_x_kernel(QrBatchArgs A) {
     const int prob = blockIdx.y;
-    if (!A.st[prob].converged) return;
+    const SolverState st = A.st[prob];
     const BatchProblem P = A.probs[prob];
+    const int k_last = st.k_run - 1;
+    const unsigned char *z =
+        (A.zf && k_last >= 0) ? A.zf + (size_t)(k_last & 1) * A.zf_len + P.zf_off : nullptr;
+    if (!st.converged && !z) return;
+    const bool even = (k_last & 1) == 0;
     const size_t total = (size_t)P.ld * (size_t)P.n_cols;
     float *x = A.x + P.mat_off;
     const float *c = A.c + P.mat_off;
     const size_t nthr = (size_t)gridDim.x * blockDim.x;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += nthr)
-        x[e] = fmaxf(__builtin_fmaf(P.step, c[e], x[e]), 0.0f);
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += nthr) {
+        const int j = (int)(e / (size_t)P.ld);
+        const int i = (int)(e - (size_t)j * (size_t)P.ld);
+        const bool flagged =
+            z && i < P.n_rows && z[(size_t)(i / kBatchTileRows) * (size_t)P.n_cols + j];
+        if (!flagged && !st.converged) continue;
+        float v = flagged ? (even ? __builtin_fmaf(-P.step, c[e], 0.0f) : 0.0f) : x[e];
+        if (st.converged) v = fmaxf(__builtin_fmaf(P.step, c[e], v), 0.0f);
+        x[e] = v;
+    }
 }
 
 int batch_pass_grid(int n_tiles) {
@@ -180,8 +203,9 @@ hipError_t launch_batch_init(const QrBatchArgs &a, hipStream_t s) {
 hipError_t launch_batch_pass(bool even, const QrBatchArgs &a, hipStream_t s) {
     if (a.n_tiles <= 0) return hipSuccess;
     const int grid = batch_pass_grid(a.n_tiles);
-    if (even) hipLaunchKernelGGL(batch_pass_kernel<true>, dim3(grid), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL(batch_pass_kernel<false>, dim3(grid), dim3(kBlock), 0, s, a);
+    auto k = a.zf ? (even ? batch_pass_kernel<true, true> : batch_pass_kernel<false, true>)
+                  : (even ? batch_pass_kernel<true, false> : batch_pass_kernel<false, false>);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 

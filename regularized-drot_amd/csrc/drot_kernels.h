@@ -126,6 +126,7 @@ struct BatchProblem {
     float step, kappa, eps;
     float v_phi1, v_phi2, v_a, v_b, v_alpha;  // warm-start constants
     int warm;
+    long long zf_off;                // zero-segment flags: [n_rt][n_cols] bytes at zf + zf_off
 };
 
 struct BatchTile {
@@ -149,6 +150,10 @@ struct QrBatchArgs {
     int n_fins;
     int n_probs;
     int k;                                     // iteration of this launch
+    // zero-segment flags (null: off): two buffers of zf_len bytes; the pass of
+    // iteration k reads buffer (k + 1) & 1 and writes buffer k & 1
+    unsigned char *zf;
+    long long zf_len;
 };
 
 // Batched launches use LPC = 16 lanes per column, 16-byte accesses (tile rows 64).

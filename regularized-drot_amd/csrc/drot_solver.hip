@@ -1053,6 +1053,8 @@ struct BatchPlan {
     int max_iters = 0;
     size_t off_state = 0, off_vec = 0, off_prow = 0, off_pcol = 0, off_pobj = 0, off_pfin = 0;
     size_t off_probs = 0, off_tiles = 0, off_fins = 0, total = 0;
+    long long zf_len = 0;  // bytes per zero-segment flag buffer (0: no flags)
+    size_t off_zf = 0;
 };
 
 drot_status plan_batch(int n, const drot_batch_problem *in, BatchPlan *out) {
@@ -1061,7 +1063,7 @@ drot_status plan_batch(int n, const drot_batch_problem *in, BatchPlan *out) {
     constexpr int cpw = kWave / kBatchLpc;
     constexpr int tc_min = kWavesPerBlock * cpw;
     int tile_off = 0, fin_off = 0;
-    long long prow_off = 0, pcol_off = 0;
+    long long prow_off = 0, pcol_off = 0, zf_off = 0;
     for (int i = 0; i < n; i++) {
         const drot_batch_problem &d = in[i];
         if (d.n_rows <= 0 || d.n_cols <= 0)
@@ -1088,8 +1090,10 @@ drot_status plan_batch(int n, const drot_batch_problem *in, BatchPlan *out) {
         P.n_ct = ceil_div(d.n_cols, tc);
         P.prow_off = prow_off;
         P.pcol_off = pcol_off;
+        P.zf_off = zf_off;
         prow_off += (long long)P.n_ct * d.n_rows;
         pcol_off += (long long)P.n_rt * d.n_cols;
+        zf_off += (long long)P.n_rt * d.n_cols;
         P.tile_off = tile_off;
         P.rb = ceil_div(d.n_rows, kFinRows);
         P.n_fin = P.rb + ceil_div(d.n_cols, kFinRows);
@@ -1137,6 +1141,11 @@ drot_status plan_batch(int n, const drot_batch_problem *in, BatchPlan *out) {
     B.off_probs = take(sizeof(BatchProblem) * (size_t)n);
     B.off_tiles = take(sizeof(BatchTile) * B.tiles.size());
     B.off_fins = take(sizeof(BatchFin) * B.fins.size());
+    // zero-segment flags for every problem (the batch keeps the HBM busy, so
+    // skipping settled segments pays at all sizes); DROT_QR_ZSKIP=0 turns them off
+    const char *zs = getenv("DROT_QR_ZSKIP");
+    B.zf_len = (zs && zs[0] == '0') ? 0 : zf_off;
+    B.off_zf = take(2 * (size_t)B.zf_len);
     B.total = off;
     *out = std::move(B);
     return DROT_OK;
@@ -1199,6 +1208,8 @@ drot_status run_batch(const BatchPlan &B, const float *c, const float *p, const 
     A.n_tiles = (int)B.tiles.size();
     A.n_fins = (int)B.fins.size();
     A.n_probs = n;
+    A.zf = B.zf_len ? reinterpret_cast<unsigned char *>(base + B.off_zf) : nullptr;
+    A.zf_len = B.zf_len;
 
     const size_t list_bytes =
         sizeof(BatchTile) * B.tiles.size() + sizeof(BatchFin) * B.fins.size();

@@ -84,3 +84,38 @@ def test_flags_keep_bits_every_tile(device, tile):
         print(f"[zero segments] tile {tile} iters={iters}", flush=True)
         _same(_solve(c, p, q, prm, device, True, tile),
               _solve(c, p, q, prm, device, False, tile))
+
+
+@pytest.mark.parametrize("max_iters,eps", [(20000, 1e-4), (151, 0.0)])
+def test_batch_flags_keep_bits(device, max_iters, eps):
+    """The batched solver flags every problem's segments (offsets into one
+    flag array, parity from the batch's iteration): same bits as without."""
+    import torch
+    from reg_drot import batch
+    shapes = [(64, 64), (130, 7), (257, 129), (1000, 800), (96, 2048), (2048, 96), (513, 511)]
+    rws = [1e-2, 1e-2, 1e-3, 1e-3, 1e-2, 1e-3, 1e-4]
+    cs, ps, qs = [], [], []
+    for i, (n, m) in enumerate(shapes):
+        cs.append(torch.from_numpy(problems.cloud_cost(n, m, dim=8, seed=40 + i)).to(device))
+        p, q = problems.uniform_marginals(n, m)
+        ps.append(torch.from_numpy(p).to(device))
+        qs.append(torch.from_numpy(q).to(device))
+    out = {}
+    old = os.environ.get("DROT_QR_ZSKIP")
+    try:
+        for zs in ("1", "0"):
+            os.environ["DROT_QR_ZSKIP"] = zs
+            print(f"[zero segments] batch of {len(shapes)} zskip={zs} max_iters={max_iters}",
+                  flush=True)
+            plans, infos = batch.quadratic_drot_batch(cs, ps, qs, 2.0, rws, max_iters, eps)
+            torch.cuda.synchronize(device)
+            out[zs] = ([x.cpu().numpy() for x in plans], infos)
+    finally:
+        if old is None:
+            os.environ.pop("DROT_QR_ZSKIP", None)
+        else:
+            os.environ["DROT_QR_ZSKIP"] = old
+    for xa, xb, ia, ib in zip(out["1"][0], out["0"][0], out["1"][1], out["0"][1]):
+        assert ia["n_iter"] == ib["n_iter"] and ia["converged"] == ib["converged"]
+        assert ia["objective_f64"] == ib["objective_f64"]
+        assert np.array_equal(xa.view(np.uint32), xb.view(np.uint32))
