@@ -9,6 +9,11 @@ value = algorithmic HBM throughput of the whole solve in GB/s:
     bytes = sum over executed iterations of 12*m*n (even k: read C and X,
             write X) or 8*m*n (odd k: read X, write X -- the even/odd -sC
             folding of core/kernel_qr.hpp is kept), divided by wall time.
+From 6M elements the pass skips the X traffic of column segments whose plan
+entries are all zero (same bits), so the bytes it actually moves are fewer than
+these algorithmic bytes: the roofline block also reports the HBM traffic per
+launch that rocprofv3 counted (profiles/qr_pass_3000x3000_pmc.json) and the
+bandwidth that traffic takes at the measured launch time (moved_GBs).
 The roofline block times the fused pass kernel itself in the library's timing
 mode over the timed steps: on sampled launches every block of the pass stores
 its start and end on the device's constant-rate clock (the launch's span is the
@@ -410,6 +415,8 @@ def main():
     odd_ev_us = odd_ev / max(odd_en, 1) * 1e3
     achieved = 12.0 * n / (even_avg_us * 1e-6) / 1e9  # even pass: the dominant kernel
     odd_gbs = 8.0 * n / (odd_avg_us * 1e-6) / 1e9
+    traffic = traffic_from_profiles()
+    moved = traffic / (even_avg_us * 1e-6) / 1e9 if traffic else None
 
     last = results[-1]
     out = {
@@ -442,9 +449,12 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(),
-            "kernel": "qr_pass_kernel<VEC=%d,LPC=%d,EVEN,ST=%d,U=%d>, %d column steps "
-                      "(12 B/element: read C, read X, write X)"
+            "traffic": traffic,
+            "moved_GBs": round(moved, 1) if moved else None,
+            "moved_frac": round(moved / HBM_PEAK_GBS, 4) if moved else None,
+            "kernel": "qr_pass_kernel<VEC=%d,LPC=%d,EVEN,ST=%d,U=%d,ZS>, %d column steps "
+                      "(12 B/element algorithmic: read C, read X, write X; with zero-segment "
+                      "flags the X traffic of all-zero segments is skipped, see traffic)"
                       % (tile[0], tile[1], tile[3], tile[4], tile[2]),
             "avg_us_even": round(even_avg_us, 3),
             "avg_us_odd": round(odd_avg_us, 3),
