@@ -1083,7 +1083,12 @@ drot_status plan_batch(int n, const drot_batch_problem *in, BatchPlan *out) {
         P.n_cols = d.n_cols;
         P.ld = d.ld;
         P.n_rt = ceil_div(d.n_rows, kBatchTileRows);
-        P.nstep = std::max(1, 4096 / (kBatchTileRows * tc_min));
+        // 16 column steps per wave (256-column tiles): with zero-segment flags
+        // a block's cost is mostly its latency chain (flag load, first data,
+        // prologue), which longer tiles spread over more columns.  Mixed
+        // 256-problem forward: 1.21 s at 4 steps, 1.04 s at 8, 0.99 s at 16
+        // (16 is the most a 16-lane column group gathers flags for).
+        P.nstep = kBatchLpc;
         const int tc = P.nstep * tc_min;
         if ((long long)tc * d.ld * 4 >= (1LL << 31))
             return fail(DROT_ERR_UNSUPPORTED, "problem " + std::to_string(i) + ": ld too large");
