@@ -72,6 +72,20 @@ def test_flags_keep_bits(device, n_rows, n_cols, iters, eps):
         assert (x <= 1e-6 * x.max()).mean() > 0.5
 
 
+@pytest.mark.parametrize("iters,warm", [(0, 1), (1, 1), (2, 1), (3, 1), (2000, 0), (157, 0)])
+def test_flags_keep_bits_short_runs_and_cold_start(device, iters, warm):
+    """The write-out after 0, 1, 2 or 3 passes (no pass, a dense even pass, ...)
+    and a cold start X = p q^T, whose first passes see no zero segment."""
+    from reg_drot import _abi
+    n_rows, n_cols = 700, 900
+    c = problems.cloud_cost(n_rows, n_cols, dim=16, seed=7)
+    p, q = problems.uniform_marginals(n_rows, n_cols)
+    prm = _abi.qr_params_from_rho(2.0, 1e-3, n_rows, n_cols, iters, 1e-4 if iters > 100 else 0.0)
+    prm.warm_start = warm
+    print(f"[zero segments] {n_rows}x{n_cols} iters={iters} warm={warm}", flush=True)
+    _same(_solve(c, p, q, prm, device, True), _solve(c, p, q, prm, device, False))
+
+
 @pytest.mark.parametrize("tile", ["4,16,7,2,2", "4,32,3", "4,64,2,2,4", "1,16,5",
                                   "1,32,16,0,2", "4,16,16,2,4", "1,64,1"])
 def test_flags_keep_bits_every_tile(device, tile):
