@@ -164,8 +164,8 @@ typedef struct drot_batch_problem {
 
 /* Fill shapes and a dense packed layout for a batch (ld = n_rows rounded up
  * to 32 floats, so every column starts on a 128-byte cache line and the
- * write-through stores of the pass write whole lines); params are left to the
- * caller.  Outputs the packed lengths. */
+ * pass's stores write whole lines); params are left to the caller.  Outputs
+ * the packed lengths. */
 drot_status drot_batch_layout(int n_problems, const int *n_rows, const int *n_cols,
                               drot_batch_problem *out, long long *mat_elems,
                               long long *p_elems, long long *q_elems);

@@ -1613,9 +1613,9 @@ drot_status cost_args(const CostPlan &B, int dim, void *ws, size_t ws_bytes, hip
 extern "C" {
 
 // Leading dimension of drot_batch_layout, in floats: columns start on 128-byte
-// lines.  With 16-byte alignment only, the sc1 write-through stores of the
-// pass wrote partial lines at every column edge and the mixed-shape batch ran
-// about 10% slower.
+// lines, so the pass writes whole lines.  With 16-byte alignment only, the
+// stores wrote partial lines at every column edge and the mixed-shape batch
+// ran about 10% slower (measured with write-through stores).
 constexpr int kBatchLdAlign = 32;
 
 drot_status drot_batch_layout(int n_problems, const int *n_rows, const int *n_cols,
