@@ -83,6 +83,9 @@ def parse():
     ap.add_argument("--profile-iters", type=int, default=0,
                     help="run one solve of exactly this many iterations and exit "
                          "(for rocprofv3 counter runs)")
+    ap.add_argument("--profile-n", type=int, default=0,
+                    help="with --profile-iters: an n x n problem of the same construction "
+                         "instead of 3000 x 3000 (the 10000 x 10000 roofline case)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run plain device copies of one 3000x3000 fp32 matrix and exit "
                          "(reference reading for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -349,6 +352,21 @@ def main():
         return
     if os.path.exists(tune.DEFAULT_TABLE):
         tune.load(tune.DEFAULT_TABLE)
+    if args.profile_iters > 0 and args.profile_n > 0:
+        n = args.profile_n
+        cn = torch.from_numpy(problems.cloud_cost(n, n, dim=16, seed=0)).to(dev)
+        pn, qn = (torch.from_numpy(a).to(dev) for a in problems.uniform_marginals(n, n))
+        xn = torch.empty_like(cn)
+        wn = _abi.qr_workspace_size(n, n)
+        wsn = torch.empty(wn, dtype=torch.uint8, device=dev)
+        pprm = _abi.qr_params_from_rho(RHO, R_WEIGHT, n, n, args.profile_iters, 0.0)
+        r = _abi.qr_solve(cn.data_ptr(), pn.data_ptr(), qn.data_ptr(), n, n, pprm,
+                          xn.data_ptr(), wsn.data_ptr(), wn,
+                          torch.cuda.current_stream(dev).cuda_stream)
+        torch.cuda.synchronize(dev)
+        print(json.dumps({"profile_n": n, "profile_iters": r.iterations_run,
+                          "residual": r.residual, "pass_tile": list(tune.tile_for(n, n))}))
+        return
     c_np = problems.cloud_cost(N_ROWS, N_COLS, dim=16, seed=0)
     p_np, q_np = problems.uniform_marginals(N_ROWS, N_COLS)
     c = torch.from_numpy(c_np).to(dev)
