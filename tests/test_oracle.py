@@ -114,7 +114,7 @@ def test_qr_1d_sorted_points_give_monotone_coupling():
 
 
 def test_warm_start_is_the_implied_iterate():
-    """Warm start == DR iterate Y0 = s 11^T (survey of pytorch/src/qr_drot_cuda_kernel.cu:45-71)."""
+    """Warm start == DR iterate Y0 = s 11^T (pytorch/src/qr_drot_cuda_kernel.cu:45-71)."""
     n_rows, n_cols = 11, 7
     s = O.step_torch(2.0, n_rows, n_cols)
     st = O.warm_start(n_rows, n_cols, float(s), dtype=np.float64)
