@@ -79,6 +79,32 @@ def test_workspace_sizes_are_modest():
         _abi.glr_workspace_size(20000, 10, 2)
 
 
+def test_zero_segment_flags_in_qr_workspace(monkeypatch):
+    """QR solves from 6M elements reserve two flag buffers of one byte per
+    (row tile, column), row tiles of vec * lanes_per_col rows; smaller ones
+    none.  DROT_QR_ZSKIP=1 / 0 turns the flags on / off for every size."""
+    from reg_drot import _abi
+
+    def ws(n, zs):
+        if zs is None:
+            monkeypatch.delenv("DROT_QR_ZSKIP", raising=False)
+        else:
+            monkeypatch.setenv("DROT_QR_ZSKIP", zs)
+        return _abi.qr_workspace_size(n, n)
+
+    def flag_bytes(n):
+        t = _abi.qr_tile_for(n, n)
+        tr = t.vec * t.lanes_per_col
+        return -(-2 * (-(-n // tr)) * n // 256) * 256  # two buffers, 256-byte aligned
+
+    for n in (2500, 3000):  # 6.25M and 9M elements
+        assert ws(n, None) - ws(n, "0") == flag_bytes(n)
+        assert ws(n, "1") == ws(n, None)
+    for n in (1000, 2048):  # below the threshold
+        assert ws(n, None) == ws(n, "0")
+        assert ws(n, "1") - ws(n, "0") == flag_bytes(n)
+
+
 def test_argument_errors_are_status_codes():
     from reg_drot import _abi
     L = lib()
