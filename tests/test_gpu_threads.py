@@ -38,8 +38,12 @@ def _gl_job(abi, c, p, q, prm, device, stream):
     return x.cpu().numpy(), r.n_iter
 
 
-def test_concurrent_solves_match_sequential(device):
+@pytest.mark.parametrize("zskip", ["0", "1"])
+def test_concurrent_solves_match_sequential(device, zskip, monkeypatch):
+    """zskip "1": the QR solves also keep zero-segment flags, which live in each
+    call's workspace like the rest of its state."""
     import torch
+    monkeypatch.setenv("DROT_QR_ZSKIP", zskip)
     from reg_drot import _abi as abi
     jobs = []
     for i, (n_rows, n_cols) in enumerate([(300, 200), (512, 512), (257, 129), (640, 480)]):
@@ -53,7 +57,7 @@ def test_concurrent_solves_match_sequential(device):
             jobs.append((_gl_job, c, p, q, prm))
     torch.cuda.synchronize(device)
     streams = [torch.cuda.Stream(device) for _ in jobs]  # default priority
-    print(f"[threads] {len(jobs)} solves, sequential then concurrent", flush=True)
+    print(f"[threads] {len(jobs)} solves, sequential then concurrent, zskip={zskip}", flush=True)
     seq = [fn(abi, c, p, q, prm, device, s) for (fn, c, p, q, prm), s in zip(jobs, streams)]
     out = [None] * len(jobs)
     errors = []
