@@ -133,3 +133,25 @@ def test_batch_flags_keep_bits(device, max_iters, eps):
         assert ia["n_iter"] == ib["n_iter"] and ia["converged"] == ib["converged"]
         assert ia["objective_f64"] == ib["objective_f64"]
         assert np.array_equal(xa.view(np.uint32), xb.view(np.uint32))
+
+
+def test_flags_keep_bits_awkward_inputs(device):
+    """Random (non-uniform) marginals and a cost with exact zeros and negative
+    entries: the recomputed fmaf(-s, c, +0) of a flagged segment meets signed
+    zeros (c = 0) and positive values (c < 0) there."""
+    from reg_drot import _abi
+    n_rows, n_cols = 777, 1029
+    rng = np.random.default_rng(11)
+    c = problems.cloud_cost(n_rows, n_cols, dim=4, seed=12)
+    c[rng.random(c.shape) < 0.02] = 0.0
+    c[rng.random(c.shape) < 0.01] *= -0.25
+    c = np.ascontiguousarray(c, dtype=np.float32)
+    p = rng.random(n_rows).astype(np.float32) + 0.5
+    q = rng.random(n_cols).astype(np.float32) + 0.5
+    p /= p.sum()
+    q /= q.sum()
+    for iters, eps in ((20000, 1e-4), (333, 0.0)):
+        prm = _abi.qr_params_from_rho(2.0, 1e-3, n_rows, n_cols, iters, eps)
+        print(f"[zero segments] awkward inputs iters={iters}", flush=True)
+        _same(_solve(c, p.astype(np.float32), q.astype(np.float32), prm, device, True),
+              _solve(c, p.astype(np.float32), q.astype(np.float32), prm, device, False))
