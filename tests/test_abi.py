@@ -179,3 +179,19 @@ def test_batch_layout_is_dense_and_line_aligned():
     assert mat == offs[-1] + 64 * 1
     assert [probs[i].p_offset for i in range(4)] == [0, 5, 69, 169] and pe == 202
     assert [probs[i].q_offset for i in range(4)] == [0, 3, 5, 12] and qe == 13
+
+
+def test_zero_segment_flags_in_batch_workspace(monkeypatch):
+    """A batch reserves two flag buffers of one byte per (64-row tile, column)
+    of every problem, at any size; DROT_QR_ZSKIP=0 removes them."""
+    from reg_drot import _abi
+    rows, cols = [5, 64, 100, 1000], [3, 300, 7, 999]
+    probs, *_ = _abi.batch_layout(rows, cols)
+    for i in range(len(rows)):
+        probs[i].params = _abi.qr_params_from_rho(2.0, 1e-3, rows[i], cols[i], 100, 1e-4)
+    monkeypatch.delenv("DROT_QR_ZSKIP", raising=False)
+    with_flags = _abi.batch_workspace_size(probs)
+    monkeypatch.setenv("DROT_QR_ZSKIP", "0")
+    without = _abi.batch_workspace_size(probs)
+    flags = 2 * sum(-(-r // 64) * c for r, c in zip(rows, cols))
+    assert with_flags - without == -(-flags // 256) * 256
