@@ -178,7 +178,8 @@ drot_status drot_qr_batch_workspace_size(int n_problems, const drot_batch_proble
  * problems (compacted as problems converge), each problem with its own
  * stopping test.  Replaces the reference's sequential loop over problems
  * (core/multiexp_qr.cu:35-55 calling quadratic_regularizer_drot_wrapper per
- * problem); each problem's result is identical to drot_qr_solve_f32 on it.
+ * problem); each problem's result is that of drot_qr_solve_f32 on it up to
+ * the grouping of the fp64 partial sums by the tile shape (same n_iter).
  * results: host array of n_problems (may be NULL). */
 drot_status drot_qr_batch_solve_f32(int n_problems, const drot_batch_problem *problems,
                                     const float *c, const float *p, const float *q, float *x,

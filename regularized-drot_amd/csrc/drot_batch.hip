@@ -3,9 +3,11 @@
 // work-list of the still-active problems.  Replaces the reference's sequential
 // host loop over problems (core/multiexp_qr.cu:35-55, which reallocates and
 // re-uploads per problem and runs each problem's 9+ launches per iteration).
-// Every tile runs the same qr_tile body as the single-problem solver and each
+// Every tile runs the same qr_tile body as the single-problem solver (64-row,
+// 256-column tiles, plain X stores, zero-segment flags per problem) and each
 // problem keeps its own SolverState, dual vectors, partials and stopping test,
-// so a problem's result is bit-identical to solving it alone.
+// so a problem's result is that of solving it alone up to the grouping of the
+// fp64 partial sums by the tile shape: the same n_iter, plans within 1e-6.
 #include "drot_finalize.h"
 #include "drot_qr_tile.h"
 

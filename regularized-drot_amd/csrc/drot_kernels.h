@@ -156,7 +156,8 @@ struct QrBatchArgs {
     long long zf_len;
 };
 
-// Batched launches use LPC = 16 lanes per column, 16-byte accesses (tile rows 64).
+// Batched launches use LPC = 16 lanes per column, 16-byte accesses (tile rows
+// 64) and LPC column steps per wave (256 columns per tile).
 constexpr int kBatchLpc = 16;
 constexpr int kBatchTileRows = kBatchLpc * 4;
 

@@ -7,8 +7,9 @@ cs[i] is an [M_i, N_i] float32 CUDA tensor, ps[i] / qs[i] its marginals
 (the reference binding's layout, pytorch/src/qr_drot_cuda_kernel.cu:89-93);
 r_weight may be one float or one per problem.  The problems are packed into
 the library's batch layout (leading dimension rounded up to 32 floats), solved by
-drot_qr_batch_solve_f32, and unpacked; each plan equals
-reg_drot.quadratic_drot on that problem alone.  PackedBatch exposes the packed
+drot_qr_batch_solve_f32, and unpacked; each plan is that of
+reg_drot.quadratic_drot on that problem alone up to the grouping of the fp64
+partial sums by the tile shape (same n_iter, plans within 1e-6).  PackedBatch exposes the packed
 buffers for callers that keep problems resident between solves.
 """
 from __future__ import annotations
