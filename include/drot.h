@@ -64,7 +64,8 @@ typedef struct drot_glr_params {
 typedef struct drot_result {
     int n_iter;            /* iteration count in the reference's convention:
                               QR: k+1 at convergence, max_iters+1 otherwise
-                              (core/drot_qr.hpp:233); GL: executed iterations */
+                              (core/drot_qr.hpp:233; INT_MAX when max_iters is
+                              INT_MAX); GL: executed iterations */
     int converged;         /* 1 if the stopping test passed                    */
     int iterations_run;    /* iterations actually executed                     */
     float objective;       /* <C, X'> of the last iteration that computed it   */

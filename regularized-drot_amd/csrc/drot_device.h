@@ -175,6 +175,12 @@ __device__ __forceinline__ int wave_uniform(int v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+// n_iter of a QR solve that never passed its stopping test: max_iters + 1
+// (core/drot_qr.hpp:233), saturated at INT_MAX for max_iters == INT_MAX.
+__host__ __device__ __forceinline__ int qr_unconverged_n_iter(int max_iters) {
+    return max_iters < 0x7fffffff ? max_iters + 1 : 0x7fffffff;
+}
+
 struct IterStart {
     bool finish;        // the solve ended with iteration k-1: do not touch X
     bool explicit_phi;  // k == 0: phi1/phi2 arrays hold the initial phi
@@ -258,7 +264,9 @@ __device__ inline IterStart iteration_prologue(const IterCtl &C, double *s_red,
     }
     if (finish && leader) {
         st->converged = conv ? 1 : 0;
-        st->n_iter = conv ? k : (C.stop_rule == kStopEveryLeq ? C.max_iters : C.max_iters + 1);
+        st->n_iter = conv ? k
+                          : (C.stop_rule == kStopEveryLeq ? C.max_iters
+                                                          : qr_unconverged_n_iter(C.max_iters));
         st->k_run = k;
         st->done = 1;
     }

@@ -28,6 +28,8 @@
 // element (C) instead of 12 (even) or 8 (odd).
 #pragma once
 
+#include <type_traits>
+
 #include "drot_kernels.h"
 
 namespace drot {
@@ -173,14 +175,16 @@ __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, 
     const float *hat2 = explicit_phi ? J.phi2 : J.bhat;
 
     // flags of this column group's segments from the previous pass: bit t is
-    // column step t (loaded one per lane, gathered by a ballot)
+    // column step t (loaded one per lane, gathered by a ballot).  A 64-lane
+    // column group can hold up to 64 steps, so its mask is 64 bits wide.
     constexpr bool zon = ZS;
-    unsigned zmask = 0;
+    using ZMask = std::conditional_t<(LPC > 32), unsigned long long, unsigned>;
+    ZMask zmask = 0;
     if (zon) {
         const int jz = col0 + rl * kColStride;
         const bool zb = rl < nstep && jz < M && J.zin[(size_t)J.rt * M + jz] != 0;
         const unsigned long long bz = __builtin_amdgcn_ballot_w64(zb);
-        zmask = (unsigned)group_bits<LPC>(bz, cs);
+        zmask = (ZMask)group_bits<LPC>(bz, cs);
     }
 
     // the matrix pointers as locals: a select between two members of J
@@ -221,7 +225,7 @@ __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, 
                 continue;
             }
             // flagged segment: even reads no X (+0), odd reads C in X's place
-            const bool z = (zmask >> t) & 1u;
+            const bool z = (zmask >> t) & 1;
             const bool live = col_ok && row_ok;
             const size_t off = (size_t)j * (size_t)ld + (size_t)row;
             const float *xs = (!EVEN && z) ? cptr : xptr;
@@ -279,7 +283,7 @@ __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, 
             const bool col_ok = j < M;
             const bool ok = col_ok && row_ok;
             const float f2 = explicit_phi ? h2a[u] : (h2a[u] + gamma) / fN;
-            if (!EVEN && ((zmask >> t) & 1u)) {
+            if (!EVEN && ((zmask >> t) & 1)) {
                 // what the even pass would have stored: X' - sC with X' = +0
 #pragma unroll
                 for (int e = 0; e < VEC; e++) xa[u][e] = __builtin_fmaf(-s, xa[u][e], 0.0f);

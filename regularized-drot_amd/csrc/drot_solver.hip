@@ -545,16 +545,18 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
     fa.cb = P.cb;
 
     // launch k (< max_iters): pass + finalize of iteration k; the last launch
-    // (k == max_iters) is a pass whose prologue only finishes iteration k-1
-    const int n_launch = in.max_iters + 1;
+    // (k == max_iters) is a pass whose prologue only finishes iteration k-1.
+    // Counted in 64 bits: max_iters may be INT_MAX.
+    const long long n_launch = (long long)in.max_iters + 1;
     const int chunk = chunk_iters(P);
     // Timing mode: events around every pass; pool halves alternate between the
     // (at most two) chunks in flight, and a half is read back before reuse.
     const bool timing = g_timing;
-    std::vector<float> pass_ms;
+    std::vector<float> pass_ms;  // per executed launch, grown chunk by chunk
     // Device-clock spans (timing mode): launches with k % 8 < 2 (one even, one
-    // odd of every eight) are sampled, each into its own [2 x grid] slot block,
-    // at most kMaxSpanBytes in total; the other launches run uninstrumented.
+    // odd of every eight) are sampled, launch k into slot block
+    // 2 (k / 8) + k % 8 of [2 x grid] each, at most kMaxSpanBytes in total;
+    // the other launches run uninstrumented.
     unsigned long long *tspan = nullptr;
     struct SpanFree {
         unsigned long long *p;
@@ -562,27 +564,30 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
             if (p) (void)hipFree(p);
         }
     } span_guard{nullptr};
-    std::vector<int> span_slot;  // per launch: sample index or -1
-    int n_samples = 0;
+    long long n_samples = 0;
+    auto span_slot = [&](long long k) -> long long {
+        if (k % 8 >= 2) return -1;
+        const long long sl = 2 * (k / 8) + k % 8;
+        return sl < n_samples ? sl : -1;
+    };
     if (timing) {
         drot_status ts = timing_pool(2 * chunk);
         if (ts != DROT_OK) return ts;
-        pass_ms.assign((size_t)std::max(n_launch, 1), 0.0f);
         constexpr size_t kMaxSpanBytes = 64u << 20;
         const size_t per = sizeof(unsigned long long) * 2 * (size_t)P.grid_pass;
-        const int cap = (int)std::max<size_t>(2, kMaxSpanBytes / per);
-        span_slot.assign((size_t)std::max(n_launch, 1), -1);
-        for (int k = 0; k < n_launch && n_samples < cap; k++)
-            if (k % 8 < 2) span_slot[(size_t)k] = n_samples++;
+        const long long cap = (long long)std::max<size_t>(2, kMaxSpanBytes / per);
+        const long long want = 2 * (n_launch / 8) + std::min<long long>(2, n_launch % 8);
+        n_samples = std::min(cap, want);
         if (n_samples > 0) {
-            DROT_HIP_CHECK(hipMalloc((void **)&tspan, per * n_samples));
+            DROT_HIP_CHECK(hipMalloc((void **)&tspan, per * (size_t)n_samples));
             span_guard.p = tspan;
-            DROT_HIP_CHECK(hipMemsetAsync(tspan, 0, per * n_samples, stream));
+            DROT_HIP_CHECK(hipMemsetAsync(tspan, 0, per * (size_t)n_samples, stream));
         }
     }
     auto span_ptr = [&](int k) -> unsigned long long * {
-        if (!tspan || k >= (int)span_slot.size() || span_slot[(size_t)k] < 0) return nullptr;
-        return tspan + (size_t)span_slot[(size_t)k] * 2 * (size_t)P.grid_pass;
+        const long long sl = span_slot(k);
+        if (!tspan || sl < 0) return nullptr;
+        return tspan + (size_t)sl * 2 * (size_t)P.grid_pass;
     };
     int chunk_start[2] = {0, 0}, chunk_n[2] = {0, 0};
     auto harvest = [&](int half) -> drot_status {
@@ -596,16 +601,17 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
         chunk_n[half] = 0;
         return DROT_OK;
     };
-    int launched = 0;
+    long long launched = 0;
     int slot = 0;
     bool pending[2] = {false, false};
     volatile int *flags = pc->pinned;
     while (launched < n_launch) {
-        const int n = std::min(chunk, n_launch - launched);
-        chunk_start[slot] = launched;
+        const int n = (int)std::min<long long>(chunk, n_launch - launched);
+        chunk_start[slot] = (int)launched;
         chunk_n[slot] = timing ? n : 0;
+        if (timing) pass_ms.resize((size_t)(launched + n), 0.0f);
         for (int t = 0; t < n; t++) {
-            const int k = launched + t;
+            const int k = (int)(launched + t);
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (timing) {
                 e0 = g_tpool.ev[(size_t)(slot * chunk + t) * 2];
@@ -662,18 +668,19 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
             drot_status st = harvest(h);
             if (st != DROT_OK) return st;
         }
-        span_us.assign((size_t)std::max(n_launch, 1), -1.0);
+        span_us.assign(pass_ms.size(), -1.0);
         if (n_samples > 0) {
             const size_t per = 2 * (size_t)P.grid_pass;
-            std::vector<unsigned long long> hsp(per * n_samples);
+            std::vector<unsigned long long> hsp(per * (size_t)n_samples);
             DROT_HIP_CHECK(hipMemcpy(hsp.data(), tspan, sizeof(unsigned long long) * hsp.size(),
                                      hipMemcpyDeviceToHost));
             int dev = 0, khz = 0;
             DROT_HIP_CHECK(hipGetDevice(&dev));
             DROT_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
             for (size_t k = 0; k < span_us.size(); k++) {
-                if (span_slot[k] < 0) continue;
-                const unsigned long long *b = hsp.data() + per * (size_t)span_slot[k];
+                const long long sl = span_slot((long long)k);
+                if (sl < 0) continue;
+                const unsigned long long *b = hsp.data() + per * (size_t)sl;
                 unsigned long long t0 = ~0ull, t1 = 0;
                 for (size_t e = 0; e < per; e += 2) {
                     if (b[e] == 0) continue;  // block did no work
@@ -685,7 +692,8 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
         }
     }
     if (res) {
-        res->n_iter = hs->done ? hs->n_iter : (P.kind == kQR ? in.max_iters + 1 : in.max_iters);
+        res->n_iter = hs->done ? hs->n_iter
+                               : (P.kind == kQR ? qr_unconverged_n_iter(in.max_iters) : in.max_iters);
         res->converged = hs->converged;
         res->iterations_run = hs->k_run;
         res->objective = (float)hs->obj;
@@ -1240,8 +1248,9 @@ drot_status run_batch(const BatchPlan &B, const float *c, const float *p, const 
     for (const auto &P : B.probs) active_bytes += (long long)P.n_rows * P.n_cols * 10;
     std::vector<char> active(n, 1);
     int n_active = n;
-    const int n_launch = B.max_iters + 1;
-    int launched = 0, slot = 0;
+    const long long n_launch = (long long)B.max_iters + 1;
+    long long launched = 0;
+    int slot = 0;
     bool pending[2] = {false, false};
     SolverState *snap[2];
     for (int h = 0; h < 2; h++)
@@ -1250,9 +1259,9 @@ drot_status run_batch(const BatchPlan &B, const float *c, const float *p, const 
     while (launched < n_launch && n_active > 0) {
         const double us = std::max(6.0, (double)active_bytes / 5.0e6);
         int chunk = std::max(2, std::min(64, (int)(1000.0 / us))) & ~1;
-        const int cnt = std::min(chunk, n_launch - launched);
+        const int cnt = (int)std::min<long long>(chunk, n_launch - launched);
         for (int t = 0; t < cnt; t++) {
-            A.k = launched + t;
+            A.k = (int)(launched + t);
             DROT_HIP_CHECK(launch_batch_pass((A.k % 2) == 0, A, stream));
             if (A.k < B.max_iters) DROT_HIP_CHECK(launch_batch_finalize(A, stream));
         }
@@ -1309,7 +1318,7 @@ drot_status run_batch(const BatchPlan &B, const float *c, const float *p, const 
             const SolverState &h = snap[0][i];
             drot_result &r = results[i];
             memset(&r, 0, sizeof(r));
-            r.n_iter = h.done ? h.n_iter : B.probs[i].max_iters + 1;
+            r.n_iter = h.done ? h.n_iter : qr_unconverged_n_iter(B.probs[i].max_iters);
             r.converged = h.converged;
             r.iterations_run = h.k_run;
             r.objective = (float)h.obj;
@@ -1839,8 +1848,9 @@ drot_status drot_qr_tile_cache_load(const char *path) {
             return fail(DROT_ERR_INVALID_ARGUMENT, std::string(path) + ":" + std::to_string(lineno) +
                                                        ": malformed line");
         const QrTile t{v, l, nt, u};
-        if (nr <= 0 || nc <= 0 || !qr_tile_supported(t) || ns < 1 || ns > 64 ||
-            (v == 4 && nr % 4 != 0))
+        // 16-byte tiles serve every row count (a column's partial last chunk
+        // is accessed element by element), as drot_qr_tile_cache_set accepts
+        if (nr <= 0 || nc <= 0 || !qr_tile_supported(t) || ns < 1 || ns > 64)
             return fail(DROT_ERR_INVALID_ARGUMENT, std::string(path) + ":" + std::to_string(lineno) +
                                                        ": unsupported tile");
         got[{nr, nc}] = TunedTile{t, ns, us};

@@ -158,6 +158,12 @@ def test_tile_cache_rejects_bad_tiles(tmp_path):
     # 16-byte tiles serve any row count (partial chunks at the column edge)
     _abi.qr_tile_cache_set(1001, 10, _abi.QrTile(4, 16, 1, 2, 2))
     assert _abi.qr_tile_for(1001, 10).as_tuple() == (4, 16, 1, 2, 2)
+    # ... and a saved table holding such a tile loads back
+    saved = tmp_path / "odd_rows.txt"
+    _abi.qr_tile_cache_save(str(saved))
+    _abi.qr_tile_cache_clear()
+    _abi.qr_tile_cache_load(str(saved))
+    assert _abi.qr_tile_for(1001, 10).as_tuple() == (4, 16, 1, 2, 2)
     _abi.qr_tile_cache_clear()
     with pytest.raises(_abi.DrotError, match="unsupported"):
         _abi.qr_tile_cache_set(1000, 10, _abi.QrTile(2, 16, 1, 2, 2))

@@ -1,0 +1,313 @@
+"""Oracle parity on the workloads the benchmark measures, at their full sizes.
+
+  * QR-DROT 3000 x 3000, C = sqdist/max of N(0, I_16) clouds, with the tuned
+    tile table loaded (the benchmark's pass tile and zero-segment flags), for
+    r_weight 1e-4, 1e-3 and 1e-2: full solves to 1e-4 against the fp32 oracle
+    (identical n_iter, plan within 1e-5) and against the FP64 oracle
+    (objective 1e-6, plan 2e-4, n_iter within 1%, residual of the returned
+    plan within 1.1 x the tolerance: the fp32-vs-fp64 bars of
+    test_gpu_fp64.py).
+  * QR-DROT 10000 x 10000, same construction, tuned 64-lane tile: iterates
+    after 2, 41 and 400 iterations (plan and potentials within 1e-5) and one
+    r_weight 1e-3 full solve (identical n_iter, plan within 1e-5).
+  * GL-DROT on the domain-adaptation task (1500 labelled source points in two
+    Gaussian classes, 1000 affine-mapped target points, 2 groups, the core
+    lambda convention of core/drot_glr.hpp:385) for r_weight 1e-3 and 5e-3:
+    full solves, identical n_iter, plan within 1e-5.
+  * The mixed batch of 256 problems (m, n ~ U[64, 2048], R^8 clouds, r_weight
+    log-uniform in [1e-4, 1e-2], the benchmark's shapes and weights): every
+    problem against its single-problem solve (identical n_iter, plan within
+    1e-6), and 8 of them against the fp32 oracle (identical n_iter, plan 1e-5).
+  * The differentiable batch of 64 problems of 512 x 512 in R^16: the
+    gradient of sum_i <W_i, X*_i> + F_i to the points, against fp64 central
+    differences (h = 1e-6) of the exact optimum on 32 coordinates (relative
+    2-norm 1e-4).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import implicit as I
+from reg_drot import problems
+
+pytestmark = pytest.mark.gpu
+
+PLAN_RTOL = 1e-5
+
+
+def _rel(a, b):
+    scale = max(float(np.abs(b).max()), 1e-30)
+    return float(np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)).max()) / scale
+
+
+def _residual(x, p, q):
+    x = x.astype(np.float64)
+    return float(np.sqrt(((p - x.sum(0)) ** 2).sum() + ((q - x.sum(1)) ** 2).sum()))
+
+
+@pytest.fixture
+def tuned_tiles(tmp_path):
+    """The measured tile table loaded for the test, the previous cache restored after."""
+    from reg_drot import tune
+    saved = tmp_path / "tiles_before.txt"
+    tune.save(str(saved))
+    tune.load(tune.DEFAULT_TABLE)
+    yield
+    tune.clear()
+    tune.load(str(saved))
+
+
+# ----------------------------------------------------------------- QR 3000^2
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("r_weight", [1e-4, 1e-3, 1e-2])
+def test_qr_3000_full_solve_with_tuned_tile(device, tuned_tiles, r_weight):
+    from reg_drot import _abi, tune
+    from test_gpu_qr import gpu_qr
+    n = 3000
+    assert tune.tile_for(n, n) == (4, 16, 7, 2, 2)
+    c = problems.cloud_cost(n, n, dim=16, seed=0)
+    p, q = problems.uniform_marginals(n, n)
+    prm = _abi.qr_params_from_rho(2.0, r_weight, n, n, 20000, 1e-4)
+    print(f"[cfg qr3000] r_weight={r_weight}", flush=True)
+    x, r, *_ = gpu_qr(c, p, q, prm, device)
+    s32, k32 = np.float32(prm.step_size), np.float32(prm.scale)
+    ref = O.qr_solve(c, p, q, s32, k32, 20000, np.float32(1e-4))
+    print(f"[cfg qr3000] gpu n_iter={r.n_iter} oracle n_iter={ref.n_iter} "
+          f"rel={_rel(x, ref.x):.2e}", flush=True)
+    assert r.converged == int(ref.converged) == 1
+    assert r.n_iter == ref.n_iter
+    assert _rel(x, ref.x) <= PLAN_RTOL
+    assert abs(r.objective_f64 - ref.objective) <= 1e-6 * abs(ref.objective)
+    f64 = np.float64
+    ref64 = O.qr_solve(c.astype(f64), p.astype(f64), q.astype(f64), f64(s32), f64(k32), 20000,
+                       1e-4, dtype=f64)
+    print(f"[cfg qr3000] fp64 n_iter={ref64.n_iter} obj rel "
+          f"{abs(r.objective_f64 - ref64.objective) / abs(ref64.objective):.2e}", flush=True)
+    assert ref64.converged
+    assert abs(r.n_iter - ref64.n_iter) <= 0.01 * ref64.n_iter + 2
+    assert abs(r.objective_f64 - ref64.objective) <= 1e-6 * abs(ref64.objective)
+    assert _rel(x, ref64.x) <= 2e-4
+    assert _residual(x, p.astype(f64), q.astype(f64)) <= 1.1e-4
+
+
+# ----------------------------------------------------------------- GL task
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("r_weight", [1e-3, 5e-3])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_gl_domain_adaptation_full_solve(device, r_weight, seed):
+    from reg_drot import _abi
+    from test_gpu_gl import gpu_gl
+    n_src, n_tgt, groups = 1500, 1000, 2
+    c, labels = problems.domain_adaptation(n_src, n_tgt, seed=seed)
+    assert (np.diff(labels) >= 0).all()  # the two classes are the two row groups
+    p = np.full(n_src, 1.0 / n_src, np.float32)
+    q = np.full(n_tgt, 1.0 / n_tgt, np.float32)
+    step = 2.0 / (n_src + n_tgt)
+    prm = _abi.glr_params_from_step(step, r_weight, n_src, n_tgt, groups, 20000, 1e-4)
+    assert np.float32(prm.lambda_) == O.gl_lambda_core(r_weight, n_src, groups)
+    print(f"[cfg gl] seed={seed} r_weight={r_weight}", flush=True)
+    x, r, *_ = gpu_gl(c, p, q, prm, device)
+    ref = O.gl_solve(c, p, q, groups, np.float32(prm.step_size), np.float32(prm.lambda_),
+                     20000, np.float32(1e-4))
+    print(f"[cfg gl] gpu n_iter={r.n_iter} oracle n_iter={ref.n_iter} "
+          f"rel={_rel(x, ref.x):.2e}", flush=True)
+    assert r.converged == int(ref.converged) == 1
+    assert r.n_iter == ref.n_iter
+    assert _rel(x, ref.x) <= PLAN_RTOL
+    assert abs(r.objective_f64 - ref.objective) <= 1e-6 * abs(ref.objective)
+
+
+# ----------------------------------------------------------------- mixed batch
+
+def _mixed_batch_spec():
+    """The mixed workload's shapes and weights (the benchmark draws them the same way)."""
+    rng = np.random.default_rng(0)
+    shapes = [(int(a), int(b)) for a, b in rng.integers(64, 2049, size=(256, 2))]
+    rws = [float(v) for v in 10 ** rng.uniform(-4, -2, size=256)]
+    return shapes, rws
+
+
+@pytest.mark.timeout(900)
+def test_mixed_batch_256_matches_single_solves_and_oracle(device):
+    import torch
+    from reg_drot import _abi, batch
+    from test_gpu_qr import gpu_qr
+    shapes, rws = _mixed_batch_spec()
+    cs_np = [problems.cloud_cost(n, m, dim=8, seed=1000 + i) for i, (n, m) in enumerate(shapes)]
+    margs = [problems.uniform_marginals(n, m) for n, m in shapes]
+    cs = [torch.from_numpy(c).to(device) for c in cs_np]
+    ps = [torch.from_numpy(pq[0]).to(device) for pq in margs]
+    qs = [torch.from_numpy(pq[1]).to(device) for pq in margs]
+    print("[cfg batch] 256 mixed problems, batched solve", flush=True)
+    pb = batch.pack(cs, ps, qs, 2.0, rws, 20000, 1e-4)
+    res = pb.solve()
+    torch.cuda.synchronize(device)
+    assert all(r.converged for r in res)
+    worst = 0.0
+    for i, (n, m) in enumerate(shapes):
+        prm = _abi.qr_params_from_rho(2.0, rws[i], n, m, 20000, 1e-4)
+        x1, r1, *_ = gpu_qr(cs_np[i], margs[i][0], margs[i][1], prm, device)
+        xb = pb.plan(i).cpu().numpy()
+        assert res[i].n_iter == r1.n_iter, (i, n, m)
+        worst = max(worst, _rel(xb, x1))
+        assert _rel(xb, x1) <= 1e-6, (i, n, m)
+        if i % 32 == 31:
+            print(f"[cfg batch] {i + 1} single solves compared, worst rel {worst:.2e}",
+                  flush=True)
+    # 8 problems against the oracle: the 4 largest and 4 others
+    order = sorted(range(len(shapes)), key=lambda i: -shapes[i][0] * shapes[i][1])
+    picks = order[:4] + [0, 77, 150, 255]
+    for i in picks:
+        n, m = shapes[i]
+        prm = _abi.qr_params_from_rho(2.0, rws[i], n, m, 20000, 1e-4)
+        ref = O.qr_solve(cs_np[i], margs[i][0], margs[i][1], np.float32(prm.step_size),
+                         np.float32(prm.scale), 20000, np.float32(1e-4))
+        xb = pb.plan(i).cpu().numpy()
+        print(f"[cfg batch] problem {i} {n}x{m} rw={rws[i]:.2e}: n_iter {res[i].n_iter} vs "
+              f"{ref.n_iter}, rel {_rel(xb, ref.x):.2e}", flush=True)
+        assert res[i].n_iter == ref.n_iter
+        assert _rel(xb, ref.x) <= PLAN_RTOL
+
+
+# ----------------------------------------------------------------- differentiable
+
+def _cost64(x, y):
+    d = ((y[:, None, :].astype(np.float64) - x[None, :, :].astype(np.float64)) ** 2).sum(-1)
+    return d / d.max()
+
+
+@pytest.mark.timeout(900)
+def test_differentiable_64x512_matches_finite_differences(device):
+    """64 problems of 512 x 512 with learnable N(0, I_16) points: the gradient of
+    sum_i <W_i, X*_i> + F_i (W_i iid N(0,1)) to the points through the batched
+    cost kernel, forward and implicit backward, against fp64 central
+    differences (h = 1e-6) of the exact optimum (oracle.implicit.newton_solve)
+    on 32 random coordinates.  The finite differences measure the exact
+    optimum's derivative, so the forward runs to 1e-7, where the fp32 plan is
+    that optimum to the gradient's accuracy (the error at the workload's 1e-4
+    is printed for reference)."""
+    import torch
+    from reg_drot import autograd, batch, cost
+    n_prob, n, dim, rw = 64, 512, 16, 1e-3
+    rng = np.random.default_rng(5)
+    xs_np = [rng.standard_normal((n, dim)).astype(np.float32) for _ in range(n_prob)]
+    ys_np = [rng.standard_normal((n, dim)).astype(np.float32) for _ in range(n_prob)]
+    ws_np = [rng.standard_normal((n, n)).astype(np.float32) for _ in range(n_prob)]
+    ps = [torch.full((n,), 1.0 / n, device=device) for _ in range(n_prob)]
+    s32, k32 = O.qr_params_torch(2.0, rw, n, n)
+    reg = I.reg_of(s32, k32)
+
+    def gpu_grads(eps):
+        xs = [torch.tensor(a, device=device, requires_grad=True) for a in xs_np]
+        ys = [torch.tensor(a, device=device, requires_grad=True) for a in ys_np]
+        pb = batch.layout(ps, ps, 2.0, rw, 200000, eps)
+        xp, objs = autograd.batch_solve_diff(cost.sqdist_packed(xs, ys, pb), pb)
+        loss = objs.sum()
+        for wi, xi in zip(ws_np, autograd.unpack(xp, pb)):
+            loss = loss + (torch.from_numpy(wi).to(device) * xi).sum()
+        loss.backward()
+        torch.cuda.synchronize(device)
+        return [a.grad.cpu().numpy() for a in xs], [b.grad.cpu().numpy() for b in ys]
+
+    print("[cfg diff] batched forward + backward at eps 1e-7 and 1e-4", flush=True)
+    gx, gy = gpu_grads(1e-7)
+    gx4, gy4 = gpu_grads(1e-4)
+    p64 = np.full(n, 1.0 / n)
+    coords = [(int(rng.integers(n_prob)), str(rng.choice(["x", "y"])), int(rng.integers(n)),
+               int(rng.integers(dim))) for _ in range(32)]
+    base = {}
+    h = 1e-6
+
+    def loss64(i, x, y):
+        c = _cost64(x, y)
+        u0, v0 = base[i]
+        xs_, *_ = I.newton_solve(c, p64, p64, reg, u0, v0)
+        return float((ws_np[i].astype(np.float64) * xs_).sum()) + I.objective(c, xs_, reg)
+
+    got, got4, fd = [], [], []
+    for t, (i, which, a, b) in enumerate(coords):
+        if i not in base:
+            _, u, v = I.exact_plan(_cost64(xs_np[i], ys_np[i]), p64, p64, s32, k32, eps=1e-7)
+            base[i] = (u, v)
+        arr0 = (xs_np if which == "x" else ys_np)[i].astype(np.float64)
+        vals = []
+        for sgn in (1.0, -1.0):
+            arr = arr0.copy()
+            arr[a, b] += sgn * h
+            xx, yy = (arr, ys_np[i]) if which == "x" else (xs_np[i], arr)
+            vals.append(loss64(i, xx, yy))
+        fd.append((vals[0] - vals[1]) / (2 * h))
+        got.append((gx if which == "x" else gy)[i][a, b])
+        got4.append((gx4 if which == "x" else gy4)[i][a, b])
+        if t % 8 == 7:
+            print(f"[cfg diff] {t + 1} finite differences done", flush=True)
+    got, got4, fd = np.array(got), np.array(got4), np.array(fd)
+    rel = float(np.linalg.norm(got - fd) / np.linalg.norm(fd))
+    rel4 = float(np.linalg.norm(got4 - fd) / np.linalg.norm(fd))
+    print(f"[cfg diff] relative error vs fp64 finite differences: eps 1e-7 {rel:.2e}, "
+          f"eps 1e-4 {rel4:.2e}", flush=True)
+    assert rel <= 1e-4
+
+
+# ----------------------------------------------------------------- QR 10000^2
+
+@pytest.fixture(scope="module")
+def qr10000():
+    c = problems.cloud_cost(10000, 10000, dim=16, seed=0)
+    p, q = problems.uniform_marginals(10000, 10000)
+    return c, p, q
+
+
+@pytest.mark.timeout(900)
+def test_qr_10000_trajectory_with_tuned_tile(device, tuned_tiles, qr10000):
+    from reg_drot import _abi, tune
+    from test_gpu_qr import gpu_qr
+    n = 10000
+    assert tune.tile_for(n, n) == (4, 64, 16, 0, 2)
+    c, p, q = qr10000
+    s32, k32 = O.qr_params_torch(2.0, 1e-3, n, n)
+    st = O.warm_start(n, n, s32)
+    done = 0
+    for iters in (2, 41, 400):
+        prm = _abi.qr_params_from_rho(2.0, 1e-3, n, n, iters, 0.0)
+        x, r, phi1, phi2, alpha = gpu_qr(c, p, q, prm, device)
+        for k in range(done, iters):
+            O.qr_step(st, k, c, p, q, s32, k32)
+        done = iters
+        print(f"[cfg qr10000] {iters} iterations: rel {_rel(x, st.x):.2e}", flush=True)
+        assert r.iterations_run == iters
+        assert _rel(x, st.x) <= PLAN_RTOL
+        assert _rel(phi1, st.phi1) <= PLAN_RTOL
+        assert _rel(phi2, st.phi2) <= PLAN_RTOL
+        assert abs(alpha - st.alpha) <= 1e-5 * max(1.0, abs(st.alpha))
+
+
+@pytest.mark.timeout(1500)
+def test_qr_10000_full_solve_with_tuned_tile(device, tuned_tiles, qr10000):
+    from reg_drot import _abi
+    from test_gpu_qr import gpu_qr
+    n = 10000
+    c, p, q = qr10000
+    prm = _abi.qr_params_from_rho(2.0, 1e-3, n, n, 20000, 1e-4)
+    x, r, *_ = gpu_qr(c, p, q, prm, device)
+    print(f"[cfg qr10000] gpu solve: n_iter={r.n_iter}; running the oracle", flush=True)
+    s32, k32 = np.float32(prm.step_size), np.float32(prm.scale)
+    st = O.warm_start(n, n, s32)
+    k, res = 0, 1e9
+    while k < 20000:
+        aux = O.qr_step(st, k, c, p, q, s32, k32)
+        if k % 2 == 0:
+            res = float(np.sqrt(aux[2] + aux[3]))
+            if res < np.float32(1e-4):
+                O.fix_x(st, c, s32)
+                break
+        k += 1
+        if k % 500 == 0:
+            print(f"[cfg qr10000] oracle at iteration {k}, residual {res:.3e}", flush=True)
+    ref_n_iter = k + 1
+    print(f"[cfg qr10000] oracle n_iter={ref_n_iter}, rel {_rel(x, st.x):.2e}", flush=True)
+    assert r.converged == 1
+    assert r.n_iter == ref_n_iter
+    assert _rel(x, st.x) <= PLAN_RTOL

@@ -156,3 +156,19 @@ def test_qr_first_iteration_bit_exact(device, r_weight, n_rows):
     want = np.asarray(ref.x, dtype=np.float32)
     assert np.isfinite(want).all()
     assert np.array_equal(x.view(np.uint32), want.view(np.uint32))
+
+
+def test_qr_int_max_iters(device):
+    """max_iters = INT_MAX (the reference's `while (k < max_iters)` allows it):
+    the solve runs to convergence with the oracle's n_iter."""
+    n_rows, n_cols = 120, 90
+    c = problems.reference_test_cost(n_rows, n_cols, seed=21)
+    p, q = problems.uniform_marginals(n_rows, n_cols)
+    big = 2 ** 31 - 1
+    prm = _abi().qr_params_from_rho(2.0, 1e-2, n_rows, n_cols, big, 1e-4)
+    x, r, *_ = gpu_qr(c, p, q, prm, device)
+    ref = O.qr_solve(c, p, q, np.float32(prm.step_size), np.float32(prm.scale), 20000,
+                     np.float32(1e-4))
+    assert r.converged == 1 and ref.converged
+    assert r.n_iter == ref.n_iter
+    assert rel_err(x, ref.x) <= PLAN_RTOL

@@ -9,7 +9,8 @@ off, so the same solve with and without them must give identical bits:
     segments then hold X' and X' - sC, the reference's stored iterate);
   - converged (fix_x runs over the written-out segments);
   - ragged row counts (partial last chunk) and every tile family, including
-    16 column steps (the most flags one column group gathers).
+    16 column steps (the most flags a 16-lane column group gathers) and 33 to
+    64 steps on 64-lane tiles (flag masks wider than 32 bits).
 The plans here are sparse from the first tens of iterations, so most segments
 take the flagged path.
 """
@@ -87,7 +88,8 @@ def test_flags_keep_bits_short_runs_and_cold_start(device, iters, warm):
 
 
 @pytest.mark.parametrize("tile", ["4,16,7,2,2", "4,32,3", "4,64,2,2,4", "1,16,5",
-                                  "1,32,16,0,2", "4,16,16,2,4", "1,64,1"])
+                                  "1,32,16,0,2", "4,16,16,2,4", "1,64,1",
+                                  "4,64,40,0,2", "4,64,64,2,4", "1,64,33"])
 def test_flags_keep_bits_every_tile(device, tile):
     from reg_drot import _abi
     n_rows, n_cols = 1003, 641

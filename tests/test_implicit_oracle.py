@@ -127,3 +127,45 @@ def test_support_rejects_fix_x_residue():
     s64 = x64 > 1e-9 * x64.max()
     assert (r32.x > 0).sum() > s64.sum()  # residues are there
     assert np.array_equal(I.support(r32.x, c32, s), s64)
+
+
+@pytest.mark.parametrize("n_rows,n_cols,rw", [(24, 20, 3e-2), (100, 80, 1e-3), (60, 90, 1e-4)])
+def test_newton_solve_is_the_exact_optimum(n_rows, n_cols, rw):
+    """newton_solve against two independent routes: the fp64 DR oracle run to
+    1e-14 (same plan to 1e-10 relative) and strong duality (primal objective
+    equals the dual value D(u, v) to 1e-12), with primal feasibility to 1e-13."""
+    from reg_drot import problems
+    c = problems.cloud_cost(n_rows, n_cols, 16, seed=n_rows + n_cols).astype(np.float64)
+    p = np.full(n_rows, 1.0 / n_rows)
+    q = np.full(n_cols, 1.0 / n_cols)
+    s, k = O.qr_params_torch(2.0, rw, n_rows, n_cols)
+    x, u, v = I.exact_plan(c, p, q, s, k)
+    reg = I.reg_of(s, k)
+    assert np.sqrt(((p - x.sum(0)) ** 2).sum() + ((q - x.sum(1)) ** 2).sum()) <= 1e-13
+    assert (x >= 0).all()
+    primal = I.objective(c, x, reg)
+    assert abs(primal - I.dual_value(u, v, c, p, q, reg)) <= 1e-12 * abs(primal)
+    xt = I.tight_solve(c, p, q, s, k, iters=400000, eps=1e-14)
+    assert np.abs(xt - x).max() <= 1e-10 * np.abs(x).max()
+
+
+def test_newton_solve_detects_a_wrong_start_cost():
+    """Starting Newton from the potentials of C and solving for C' (C with one
+    support entry raised) moves the plan: the solver does not just return its
+    start, and the new plan is feasible."""
+    from reg_drot import problems
+    n_rows, n_cols = 40, 30
+    c = problems.cloud_cost(n_rows, n_cols, 16, seed=5).astype(np.float64)
+    p = np.full(n_rows, 1.0 / n_rows)
+    q = np.full(n_cols, 1.0 / n_cols)
+    # a strong regularization gives a dense support (a tree-shaped support
+    # would fix the plan by the marginals alone)
+    s, k = O.qr_params_torch(2.0, 0.5, n_rows, n_cols)
+    x, u, v = I.exact_plan(c, p, q, s, k)
+    assert (x > 0).mean() > 0.5
+    c2 = c.copy()
+    j, i = np.unravel_index(int(np.argmax(x)), x.shape)  # an entry of the support
+    c2[j, i] += 1e-3
+    x2, *_ = I.newton_solve(c2, p, q, I.reg_of(s, k), u, v)
+    assert np.abs(x2 - x).max() > 1e-8
+    assert np.sqrt(((p - x2.sum(0)) ** 2).sum() + ((q - x2.sum(1)) ** 2).sum()) <= 1e-13
