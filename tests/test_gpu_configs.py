@@ -18,10 +18,11 @@
     log-uniform in [1e-4, 1e-2], the benchmark's shapes and weights): every
     problem against its single-problem solve (identical n_iter, plan within
     1e-6), and 8 of them against the fp32 oracle (identical n_iter, plan 1e-5).
-  * The differentiable batch of 64 problems of 512 x 512 in R^16: the
-    gradient of sum_i <W_i, X*_i> + F_i to the points, against fp64 central
-    differences (h = 1e-6) of the exact optimum on 32 coordinates (relative
-    2-norm 1e-4).
+  * The differentiable batch of 64 problems of 512 x 512 in R^16, and 8
+    problems of the mixed workload: the gradient of sum_i <W_i, X*_i> + F_i to
+    the points, against fp64 central differences (h = 1e-6) of the exact
+    optimum on 32 coordinates (relative 2-norm 1e-4 at a forward tolerance of
+    1e-7; the errors at 1e-4 .. 1e-6 are printed).
 """
 import numpy as np
 import pytest
@@ -171,38 +172,40 @@ def test_mixed_batch_256_matches_single_solves_and_oracle(device):
         assert _rel(xb, ref.x) <= PLAN_RTOL
 
 
-# ----------------------------------------------------------------- differentiable
+# ----------------------------------------------------------------- gradients
 
 def _cost64(x, y):
     d = ((y[:, None, :].astype(np.float64) - x[None, :, :].astype(np.float64)) ** 2).sum(-1)
     return d / d.max()
 
 
-@pytest.mark.timeout(900)
-def test_differentiable_64x512_matches_finite_differences(device):
-    """64 problems of 512 x 512 with learnable N(0, I_16) points: the gradient of
-    sum_i <W_i, X*_i> + F_i (W_i iid N(0,1)) to the points through the batched
-    cost kernel, forward and implicit backward, against fp64 central
-    differences (h = 1e-6) of the exact optimum (oracle.implicit.newton_solve)
-    on 32 random coordinates.  The finite differences measure the exact
-    optimum's derivative, so the forward runs to 1e-7, where the fp32 plan is
-    that optimum to the gradient's accuracy (the error at the workload's 1e-4
-    is printed for reference)."""
+# forward tolerances at which the batched gradient is compared with the exact
+# optimum's finite differences; GRAD_EPS is the one the bar applies to
+FWD_EPS = (1e-4, 1e-5, 1e-6, 1e-7)
+GRAD_EPS = 1e-7
+
+
+def _fd_check(device, tag, shapes, rws, dim, seed, n_coords):
+    """Gradient of sum_i <W_i, X*_i> + F_i to the points of a batch (cost
+    kernel, batched forward, implicit backward) at each forward tolerance of
+    FWD_EPS, against fp64 central differences (h = 1e-6) of the exact optimum
+    (oracle.implicit.newton_solve, started from the potentials of a
+    single-problem GPU solve) on n_coords random coordinates.  Returns
+    {eps: relative 2-norm error}."""
     import torch
-    from reg_drot import autograd, batch, cost
-    n_prob, n, dim, rw = 64, 512, 16, 1e-3
-    rng = np.random.default_rng(5)
-    xs_np = [rng.standard_normal((n, dim)).astype(np.float32) for _ in range(n_prob)]
-    ys_np = [rng.standard_normal((n, dim)).astype(np.float32) for _ in range(n_prob)]
-    ws_np = [rng.standard_normal((n, n)).astype(np.float32) for _ in range(n_prob)]
-    ps = [torch.full((n,), 1.0 / n, device=device) for _ in range(n_prob)]
-    s32, k32 = O.qr_params_torch(2.0, rw, n, n)
-    reg = I.reg_of(s32, k32)
+    from reg_drot import _abi, autograd, batch, cost
+    from test_gpu_qr import gpu_qr
+    rng = np.random.default_rng(seed)
+    xs_np = [rng.standard_normal((n, dim)).astype(np.float32) for n, _ in shapes]
+    ys_np = [rng.standard_normal((m, dim)).astype(np.float32) for _, m in shapes]
+    ws_np = [rng.standard_normal((m, n)).astype(np.float32) for n, m in shapes]
+    ps = [torch.full((n,), 1.0 / n, device=device) for n, _ in shapes]
+    qs = [torch.full((m,), 1.0 / m, device=device) for _, m in shapes]
 
     def gpu_grads(eps):
         xs = [torch.tensor(a, device=device, requires_grad=True) for a in xs_np]
         ys = [torch.tensor(a, device=device, requires_grad=True) for a in ys_np]
-        pb = batch.layout(ps, ps, 2.0, rw, 200000, eps)
+        pb = batch.layout(ps, qs, 2.0, rws, 200000, eps)
         xp, objs = autograd.batch_solve_diff(cost.sqdist_packed(xs, ys, pb), pb)
         loss = objs.sum()
         for wi, xi in zip(ws_np, autograd.unpack(xp, pb)):
@@ -211,26 +214,41 @@ def test_differentiable_64x512_matches_finite_differences(device):
         torch.cuda.synchronize(device)
         return [a.grad.cpu().numpy() for a in xs], [b.grad.cpu().numpy() for b in ys]
 
-    print("[cfg diff] batched forward + backward at eps 1e-7 and 1e-4", flush=True)
-    gx, gy = gpu_grads(1e-7)
-    gx4, gy4 = gpu_grads(1e-4)
-    p64 = np.full(n, 1.0 / n)
-    coords = [(int(rng.integers(n_prob)), str(rng.choice(["x", "y"])), int(rng.integers(n)),
-               int(rng.integers(dim))) for _ in range(32)]
+    print(f"[{tag}] batched forward + backward at eps {FWD_EPS}", flush=True)
+    grads = {eps: gpu_grads(eps) for eps in FWD_EPS}
+    coords = []
+    for _ in range(n_coords):
+        i = int(rng.integers(len(shapes)))
+        which = str(rng.choice(["x", "y"]))
+        a = int(rng.integers(shapes[i][0] if which == "x" else shapes[i][1]))
+        coords.append((i, which, a, int(rng.integers(dim))))
     base = {}
     h = 1e-6
 
+    def exact(i):
+        if i not in base:
+            n, m = shapes[i]
+            c = _cost64(xs_np[i], ys_np[i])
+            prm = _abi.qr_params_from_rho(2.0, rws[i], n, m, 200000, 1e-7)
+            _, _, phi1, phi2, _ = gpu_qr(c.astype(np.float32), *problems.uniform_marginals(n, m),
+                                         prm, device)
+            s = float(np.float32(prm.step_size))
+            reg = I.reg_of(prm.step_size, prm.scale)
+            p64, q64 = np.full(n, 1.0 / n), np.full(m, 1.0 / m)
+            _, u, v, gn = I.newton_solve(c, p64, q64, reg, phi1 / s, phi2 / s)
+            assert gn <= 1e-13, gn
+            base[i] = (u, v, reg, p64, q64)
+        return base[i]
+
     def loss64(i, x, y):
+        u0, v0, reg, p64, q64 = exact(i)
         c = _cost64(x, y)
-        u0, v0 = base[i]
-        xs_, *_ = I.newton_solve(c, p64, p64, reg, u0, v0)
+        xs_, _, _, gn = I.newton_solve(c, p64, q64, reg, u0, v0)
+        assert gn <= 1e-13, gn
         return float((ws_np[i].astype(np.float64) * xs_).sum()) + I.objective(c, xs_, reg)
 
-    got, got4, fd = [], [], []
+    fd = []
     for t, (i, which, a, b) in enumerate(coords):
-        if i not in base:
-            _, u, v = I.exact_plan(_cost64(xs_np[i], ys_np[i]), p64, p64, s32, k32, eps=1e-7)
-            base[i] = (u, v)
         arr0 = (xs_np if which == "x" else ys_np)[i].astype(np.float64)
         vals = []
         for sgn in (1.0, -1.0):
@@ -239,16 +257,38 @@ def test_differentiable_64x512_matches_finite_differences(device):
             xx, yy = (arr, ys_np[i]) if which == "x" else (xs_np[i], arr)
             vals.append(loss64(i, xx, yy))
         fd.append((vals[0] - vals[1]) / (2 * h))
-        got.append((gx if which == "x" else gy)[i][a, b])
-        got4.append((gx4 if which == "x" else gy4)[i][a, b])
         if t % 8 == 7:
-            print(f"[cfg diff] {t + 1} finite differences done", flush=True)
-    got, got4, fd = np.array(got), np.array(got4), np.array(fd)
-    rel = float(np.linalg.norm(got - fd) / np.linalg.norm(fd))
-    rel4 = float(np.linalg.norm(got4 - fd) / np.linalg.norm(fd))
-    print(f"[cfg diff] relative error vs fp64 finite differences: eps 1e-7 {rel:.2e}, "
-          f"eps 1e-4 {rel4:.2e}", flush=True)
-    assert rel <= 1e-4
+            print(f"[{tag}] {t + 1} finite differences done", flush=True)
+    fd = np.array(fd)
+    out = {}
+    for eps, (gx, gy) in grads.items():
+        got = np.array([(gx if w == "x" else gy)[i][a, b] for i, w, a, b in coords])
+        out[eps] = float(np.linalg.norm(got - fd) / np.linalg.norm(fd))
+    print(f"[{tag}] relative error vs fp64 finite differences by forward tolerance: "
+          + ", ".join(f"{e:g}: {v:.2e}" for e, v in out.items()), flush=True)
+    return out
+
+
+@pytest.mark.timeout(900)
+def test_differentiable_64x512_matches_finite_differences(device):
+    """64 problems of 512 x 512 with learnable N(0, I_16) points, r_weight
+    1e-3, on 32 coordinates.  The finite differences measure the exact
+    optimum's derivative, which an fp32 plan converged to 1e-4 matches only to
+    a few percent (its marginals are off by up to 1e-4); the bar (1e-4) applies
+    at the forward tolerance GRAD_EPS."""
+    err = _fd_check(device, "cfg diff", [(512, 512)] * 64, [1e-3] * 64, 16, 5, 32)
+    assert err[GRAD_EPS] <= 1e-4
+
+
+@pytest.mark.timeout(900)
+def test_mixed_batch_gradient_matches_finite_differences(device):
+    """The mixed workload's construction (m, n ~ U[64, 2048], R^8, r_weight
+    log-uniform in [1e-4, 1e-2]) on 8 of its problems, 32 coordinates."""
+    shapes, rws = _mixed_batch_spec()
+    pick = [3, 40, 77, 101, 150, 199, 230, 255]
+    err = _fd_check(device, "cfg mixed grad", [shapes[i] for i in pick], [rws[i] for i in pick],
+                    8, 9, 32)
+    assert err[GRAD_EPS] <= 1e-4
 
 
 # ----------------------------------------------------------------- QR 10000^2
