@@ -132,6 +132,24 @@ drot_status drot_qr_solve_f32(const float *c, const float *p, const float *q, in
                               void *workspace, size_t workspace_bytes, void *stream,
                               drot_result *result);
 
+/* Allocation-free variant of drot_qr_solve_f32.  It has no counterpart in the
+ * reference; it is for callers that cannot allocate: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  It calls no
+ * hipMalloc, hipFree, hipHostMalloc or other allocating routine: the solver's
+ * workspace is a static device array of DROT_NOALLOC_SCRATCH_BYTES that every
+ * device holds from the library's first use there until the process ends, and
+ * the host polls the done flag through pageable memory (timing mode is
+ * ignored).  Shapes whose workspace (drot_qr_workspace_size) exceeds that size
+ * return DROT_ERR_WORKSPACE_TOO_SMALL.  All calls on one device share that
+ * device's array and it keeps the last call's contents (each solve
+ * initialises what it reads): overlapping calls on one device would race, so
+ * the library makes them wait for each other within a process.  Same results
+ * as drot_qr_solve_f32, bit for bit. */
+#define DROT_NOALLOC_SCRATCH_BYTES (32u << 20)
+drot_status drot_qr_solve_noalloc_f32(const float *c, const float *p, const float *q,
+                                      int n_rows, int n_cols, const drot_qr_params *params,
+                                      float *x, void *stream, drot_result *result);
+
 /* GL-DROT solve on device memory.  Replaces group_lasso_regularizer_drot<float>
  * (core/drot_glr.hpp:242-355) as called by group_lasso_regularizer_drot_torch_float32
  * (pytorch/src/glr_drot_cuda_kernel.cu:13-86). */

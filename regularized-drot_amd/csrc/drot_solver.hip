@@ -425,12 +425,18 @@ struct SolveIn {
     int warm;
 };
 
+// noalloc: the solve allocates nothing (no pinned snapshot slots, no events,
+// no timing buffers): after every chunk the done flag is copied into pageable
+// host memory, which returns once the chunk has finished, and timing mode is
+// ignored.  The GPU then idles for the host's turnaround once per chunk.
 drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
-                drot_result *res) {
+                drot_result *res, bool noalloc = false) {
     DevPtrs d = carve(P, ws);
     PollCtx *pc = nullptr;
-    drot_status ps = poll_ctx(&pc);
-    if (ps != DROT_OK) return ps;
+    if (!noalloc) {
+        drot_status ps = poll_ctx(&pc);
+        if (ps != DROT_OK) return ps;
+    }
 
     InitArgs ia{};
     ia.x = in.x;
@@ -551,7 +557,7 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
     const int chunk = chunk_iters(P);
     // Timing mode: events around every pass; pool halves alternate between the
     // (at most two) chunks in flight, and a half is read back before reuse.
-    const bool timing = g_timing;
+    const bool timing = g_timing && !noalloc;
     std::vector<float> pass_ms;  // per executed launch, grown chunk by chunk
     // Device-clock spans (timing mode): launches with k % 8 < 2 (one even, one
     // odd of every eight) are sampled, launch k into slot block
@@ -604,7 +610,7 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
     long long launched = 0;
     int slot = 0;
     bool pending[2] = {false, false};
-    volatile int *flags = pc->pinned;
+    volatile int *flags = noalloc ? nullptr : pc->pinned;
     while (launched < n_launch) {
         const int n = (int)std::min<long long>(chunk, n_launch - launched);
         chunk_start[slot] = (int)launched;
@@ -635,6 +641,14 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
             }
         }
         launched += n;
+        if (noalloc) {
+            int done = 0;
+            DROT_HIP_CHECK(hipMemcpyAsync(&done, &d.st->done, sizeof(int),
+                                          hipMemcpyDeviceToHost, stream));
+            DROT_HIP_CHECK(hipStreamSynchronize(stream));
+            if (done) break;
+            continue;
+        }
         DROT_HIP_CHECK(hipMemcpyAsync((void *)&flags[slot], &d.st->done, sizeof(int),
                                       hipMemcpyDeviceToHost, stream));
         DROT_HIP_CHECK(hipEventRecord(pc->ev[slot], stream));
@@ -658,7 +672,8 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
         DROT_HIP_CHECK(launch_fix_x(in.x, in.c, in.step,
                                     (size_t)P.n_rows * (size_t)P.n_cols, d.st, stream));
     }
-    SolverState *hs = reinterpret_cast<SolverState *>(pc->pinned + 16);
+    SolverState hs_local;
+    SolverState *hs = noalloc ? &hs_local : reinterpret_cast<SolverState *>(pc->pinned + 16);
     DROT_HIP_CHECK(hipMemcpyAsync(hs, d.st, sizeof(SolverState), hipMemcpyDeviceToHost,
                                   stream));
     DROT_HIP_CHECK(hipStreamSynchronize(stream));
@@ -854,6 +869,68 @@ drot_status drot_qr_solve_f32(const float *c, const float *p, const float *q, in
     SolveIn in{c, p, q, x, prm->step_size, prm->scale, 0.0f, prm->max_iters, prm->eps,
                prm->warm_start};
     return run(P, in, workspace, static_cast<hipStream_t>(stream), result);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------- allocation-free QR solve
+
+// Device scratch of drot_qr_solve_noalloc_f32, in place of the caller's
+// workspace: DROT_NOALLOC_SCRATCH_BYTES per device, part of this library's
+// code object, so each device holds its own copy from the library's first use
+// there until the process ends.  Host code never passes `scratch` itself to a
+// kernel (in host code it names a host-side stand-in): it passes the device
+// address that hipGetSymbolAddress returned on that device.
+static __device__ __align__(256) unsigned char scratch[DROT_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+
+// Per device: the address of `scratch` there (looked up on the first call on
+// the device, with the device current) and a lock that makes the calls on one
+// device take the shared scratch one at a time.
+struct NoAllocDevice {
+    std::mutex mu;
+    void *ptr = nullptr;
+};
+NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+
+}  // namespace
+
+extern "C" {
+
+drot_status drot_qr_solve_noalloc_f32(const float *c, const float *p, const float *q,
+                                      int n_rows, int n_cols, const drot_qr_params *prm,
+                                      float *x, void *stream, drot_result *result) {
+    drot_status s = check_common(c, p, q, n_rows, n_cols, x, prm ? prm->max_iters : 0);
+    if (s != DROT_OK) return s;
+    if (!prm) return fail(DROT_ERR_INVALID_ARGUMENT, "params is NULL");
+    if (!(prm->scale > 0.0f)) return fail(DROT_ERR_INVALID_ARGUMENT, "scale must be > 0");
+    const Plan P = plan_qr(n_rows, n_cols, aligned16(c) && aligned16(x));
+    if (P.total > (size_t)DROT_NOALLOC_SCRATCH_BYTES)
+        return fail(DROT_ERR_WORKSPACE_TOO_SMALL,
+                    "a " + std::to_string(n_rows) + "x" + std::to_string(n_cols) +
+                        " solve needs " + std::to_string(P.total) +
+                        " bytes of scratch; the allocation-free solve holds " +
+                        std::to_string((size_t)DROT_NOALLOC_SCRATCH_BYTES) +
+                        " (use drot_qr_solve_f32 with a workspace)");
+    int dev = 0;
+    DROT_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices)
+        return fail(DROT_ERR_UNSUPPORTED, "device index " + std::to_string(dev) +
+                                              " beyond the allocation-free solve's table");
+    NoAllocDevice &D = g_noalloc[dev];
+    std::lock_guard<std::mutex> lk(D.mu);
+    if (!D.ptr) {
+        void *ptr = nullptr;
+        DROT_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(DROT_ERR_HIP, "hipGetSymbolAddress returned a null scratch address");
+        D.ptr = ptr;
+    }
+    SolveIn in{c, p, q, x, prm->step_size, prm->scale, 0.0f, prm->max_iters, prm->eps,
+               prm->warm_start};
+    return run(P, in, D.ptr, static_cast<hipStream_t>(stream), result, true);
 }
 
 drot_status drot_glr_solve_f32(const float *c, const float *p, const float *q, int n_rows,

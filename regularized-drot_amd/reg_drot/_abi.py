@@ -32,8 +32,12 @@ EXPORTS = (
     "drot_sqdist_workspace_size", "drot_sqdist_f32", "drot_sqdist_backward_f32",
     "drot_qr_tile_for", "drot_qr_autotune_workspace_size", "drot_qr_autotune_f32",
     "drot_qr_tile_cache_set", "drot_qr_tile_cache_save", "drot_qr_tile_cache_load",
-    "drot_qr_tile_cache_clear",
+    "drot_qr_tile_cache_clear", "drot_qr_solve_noalloc_f32",
 )
+
+# DROT_NOALLOC_SCRATCH_BYTES of include/drot.h: the device scratch of the
+# allocation-free solve
+NOALLOC_SCRATCH_BYTES = 32 << 20
 
 DROT_OK = 0
 ABI_VERSION = 2  # DROT_ABI_VERSION of the include/drot.h these prototypes follow
@@ -134,6 +138,8 @@ def load(path: str = LIB_PATH):
                                       vp, P(Result)]
     lib.drot_glr_solve_f32.argtypes = [vp, vp, vp, c_int, c_int, P(GlrParams), vp, vp, c_size,
                                        vp, P(Result)]
+    lib.drot_qr_solve_noalloc_f32.argtypes = [vp, vp, vp, c_int, c_int, P(QrParams), vp, vp,
+                                              P(Result)]
     lib.drot_export_duals_f32.argtypes = [c_int, c_int, c_int, c_int, vp, c_size, vp, vp,
                                           P(c_float), vp]
     fp = P(c_float)
@@ -193,6 +199,7 @@ def load(path: str = LIB_PATH):
                  "drot_glr_params_from_rho", "drot_glr_params_from_step",
                  "drot_qr_workspace_size", "drot_glr_workspace_size",
                  "drot_qr_solve_f32", "drot_glr_solve_f32", "drot_export_duals_f32",
+                 "drot_qr_solve_noalloc_f32",
                  "drot_batch_layout", "drot_qr_batch_workspace_size", "drot_qr_batch_solve_f32"):
         getattr(lib, name).restype = c_int
     _lib = lib
@@ -255,6 +262,15 @@ def qr_solve(c, p, q, n_rows, n_cols, prm: QrParams, x, ws, ws_bytes, stream) ->
     r = Result()
     check(load().drot_qr_solve_f32(c, p, q, n_rows, n_cols, ctypes.byref(prm), x, ws,
                                    ws_bytes, stream, ctypes.byref(r)), "drot_qr_solve_f32")
+    return r
+
+
+def qr_solve_noalloc(c, p, q, n_rows, n_cols, prm: QrParams, x, stream) -> Result:
+    """drot_qr_solve_noalloc_f32: no workspace argument, nothing allocated."""
+    r = Result()
+    check(load().drot_qr_solve_noalloc_f32(c, p, q, n_rows, n_cols, ctypes.byref(prm), x,
+                                           stream, ctypes.byref(r)),
+          "drot_qr_solve_noalloc_f32")
     return r
 
 
