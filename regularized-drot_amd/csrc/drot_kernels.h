@@ -45,6 +45,13 @@ struct GlPassArgs {
     int lane_per_group; // 1: one lane sums a group serially (small groups)
     float step, lambda;
     IterCtl ctl;
+    // long columns (GlTile::split): row chunks of kBlock*VEC*RCH rows; group g
+    // owns the partial-norm rows gseg[g] .. gseg[g+1]-1 of pnorm
+    // ([n_seg][n_cols]), one per chunk it touches from chunk gclo[g] on
+    int n_rt;           // row chunks (split) / 1
+    double *pnorm;
+    const int *gseg;    // [n_groups + 1]
+    const int *gclo;    // [n_groups]
 };
 
 // Arguments of the dual / aux update (shared by QR and GL).
@@ -80,9 +87,11 @@ struct QrTile {
 };
 
 struct GlTile {
-    int vec;   // 1 or 4
-    int rch;   // row chunks per thread (per lane for the wave-per-column kernel)
-    int wave;  // 1: one wave per column (gl_wave_kernel), n_rows <= 2048
+    int vec;        // 1 or 4
+    int rch;        // row chunks per thread (per lane for the wave-per-column kernel)
+    int wave;       // 1: one wave per column (gl_wave_kernel), n_rows <= 2048
+    int split = 0;  // 1: columns split into row chunks over blocks (any n_rows):
+                    // gl_split_norm_kernel + gl_split_apply_kernel per iteration
 };
 
 hipError_t launch_init(const InitArgs &a, hipStream_t s);

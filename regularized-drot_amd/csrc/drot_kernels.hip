@@ -489,6 +489,227 @@ __global__ __launch_bounds__(kBlock) void gl_wave_kernel(GlPassArgs A) {
     if (A.ctl.tspan) record_span(A.ctl.tspan, t0);
 }
 
+// ------------------------------------------------- GL pass for long columns
+//
+// Columns too long for one block (the fused kernels above keep a whole column
+// in registers and LDS) are cut into row chunks of R = kBlock*VEC*RCH rows,
+// block (rc, ct) taking chunk rc of the columns of tile ct, and the group
+// norms are completed across chunks between two launches, as the reference's
+// large-group chain does between update_x and apply_norm
+// (core/kernel_glr.hpp:163-399, core/drot_glr.hpp:145-240):
+//   gl_split_norm_kernel   iteration prologue; y = max(X + phi1 + phi2 - sC, 0)
+//                          for the chunk, and per (group segment, column) the
+//                          fp64 sum of y^2 over the group's rows in the chunk;
+//                          X is not written
+//   gl_split_apply_kernel  the same y again (X is unchanged, so the bits are
+//                          the same), the group norm as the sum of its
+//                          segments in chunk order, X := y * scale, and the
+//                          row / column / objective partials of the fused pass
+// 20 bytes per element (read X, C twice; write X) against the fused
+// kernels' 12, the same as the reference's update_x + apply_norm.
+
+template <int VEC, int RCH>
+struct SplitChunk {
+    static constexpr int R = kBlock * VEC * RCH;
+};
+
+// Loads of one chunk's column j: y = max((x + f1) + f2 - s c, 0) per element
+// (rows past n_rows give 0).  Rows of chunk element (r, e): r0 + (r*kBlock + t)*VEC + e.
+template <int VEC, int RCH>
+__device__ __forceinline__ void split_prox(const GlPassArgs &A, int j, int r0, float f2,
+                                           const float (&f1)[RCH][VEC], float (&y)[RCH][VEC],
+                                           float (&cv)[RCH][VEC]) {
+    const int N = A.n_rows;
+    const float s = A.step;
+    const float *xc = A.x + (size_t)j * N;
+    const float *cc = A.c + (size_t)j * N;
+    float xv[RCH][VEC];
+#pragma unroll
+    for (int r = 0; r < RCH; r++) {
+        const int i0 = r0 + (r * kBlock + (int)threadIdx.x) * VEC;
+        if (i0 < N) {
+            VecIO<VEC>::load(xc + i0, xv[r]);
+            VecIO<VEC>::load(cc + i0, cv[r]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < VEC; e++) { xv[r][e] = 0.0f; cv[r][e] = 0.0f; }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RCH; r++) {
+#pragma unroll
+        for (int e = 0; e < VEC; e++) {
+            const float tt = (xv[r][e] + f1[r][e]) + f2;
+            y[r][e] = fmaxf(__builtin_fmaf(-s, cv[r][e], tt), 0.0f);
+        }
+    }
+}
+
+// Row potentials and groups of the chunk's rows (issued before the prologue's
+// reductions in the norm kernel).
+template <int VEC, int RCH>
+__device__ __forceinline__ void split_rows(const GlPassArgs &A, bool explicit_phi, int r0,
+                                           float (&h1)[RCH][VEC], int (&gid)[RCH][VEC]) {
+    const float *hsrc = explicit_phi ? A.phi1 : A.ahat;
+#pragma unroll
+    for (int r = 0; r < RCH; r++) {
+        const int i0 = r0 + (r * kBlock + (int)threadIdx.x) * VEC;
+#pragma unroll
+        for (int e = 0; e < VEC; e++) { h1[r][e] = 0.0f; gid[r][e] = 0; }
+        if (i0 < A.n_rows) {
+            VecIO<VEC>::load(hsrc + i0, h1[r]);
+            load_ints<VEC>(A.row_group + i0, gid[r]);
+        }
+    }
+}
+
+template <int VEC, int RCH>
+__global__ __launch_bounds__(kBlock) void gl_split_norm_kernel(GlPassArgs A) {
+    constexpr int R = SplitChunk<VEC, RCH>::R;
+    __shared__ float s_y[R];
+    __shared__ double s_red[3 * kWavesPerBlock];
+    if (A.ctl.st->done) return;
+    const int rc = blockIdx.x % A.n_rt;
+    const int ct = blockIdx.x / A.n_rt;
+    const int N = A.n_rows, M = A.n_cols;
+    const int r0 = rc * R;
+    const int r1 = min(r0 + R, N);
+    const bool explicit_phi = A.ctl.k == 0;
+    float h1[RCH][VEC];
+    int gid[RCH][VEC];
+    split_rows<VEC, RCH>(A, explicit_phi, r0, h1, gid);
+    const int g_first = A.row_group[r0];
+    const int g_last = A.row_group[r1 - 1];
+    const IterStart is = iteration_prologue(A.ctl, s_red, blockIdx.x == 0);
+    if (is.finish) return;
+    const float gamma = is.gamma;
+    const float fM = (float)M, fN = (float)N;
+    float f1[RCH][VEC];
+#pragma unroll
+    for (int r = 0; r < RCH; r++)
+#pragma unroll
+        for (int e = 0; e < VEC; e++) f1[r][e] = explicit_phi ? h1[r][e] : (h1[r][e] + gamma) / fM;
+    const int ng = g_last - g_first + 1;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = wave_uniform(threadIdx.x / kWave);
+    const int j0 = ct * A.tc;
+    const int j1 = min(j0 + A.tc, M);
+    for (int j = j0; j < j1; j++) {
+        const float h2 = explicit_phi ? A.phi2[j] : A.bhat[j];
+        const float f2 = explicit_phi ? h2 : (h2 + gamma) / fN;
+        float y[RCH][VEC], cv[RCH][VEC];
+        split_prox<VEC, RCH>(A, j, r0, f2, f1, y, cv);
+#pragma unroll
+        for (int r = 0; r < RCH; r++)
+#pragma unroll
+            for (int e = 0; e < VEC; e++) s_y[(r * kBlock + (int)threadIdx.x) * VEC + e] = y[r][e];
+        __syncthreads();
+        // per group touching the chunk: fp64 sum of y^2 over its rows here
+        if (A.lane_per_group) {
+            for (int gl = threadIdx.x; gl < ng; gl += kBlock) {
+                const int g = g_first + gl;
+                const int lo = max(A.bounds[g], r0) - r0, hi = min(A.bounds[g + 1], r1) - r0;
+                double n2 = 0.0;
+                for (int i = lo; i < hi; i++) n2 += (double)s_y[i] * (double)s_y[i];
+                A.pnorm[(size_t)(A.gseg[g] + rc - A.gclo[g]) * M + j] = n2;
+            }
+        } else {
+            for (int gl = w; gl < ng; gl += kWavesPerBlock) {
+                const int g = g_first + gl;
+                const int lo = max(A.bounds[g], r0) - r0, hi = min(A.bounds[g + 1], r1) - r0;
+                double n2 = 0.0;
+                for (int i = lo + lane; i < hi; i += kWave) n2 += (double)s_y[i] * (double)s_y[i];
+                n2 = wave_sum(n2);
+                if (lane == 0) A.pnorm[(size_t)(A.gseg[g] + rc - A.gclo[g]) * M + j] = n2;
+            }
+        }
+        __syncthreads();  // s_y is rewritten by the next column
+    }
+}
+
+template <int VEC, int RCH>
+__global__ __launch_bounds__(kBlock) void gl_split_apply_kernel(GlPassArgs A) {
+    constexpr int R = SplitChunk<VEC, RCH>::R;
+    __shared__ float s_scale[R];
+    __shared__ double s_red[kWavesPerBlock];
+    // the norm kernel of this iteration ran the prologue: done is set when the
+    // solve ended with iteration k-1, and gamma is that of iteration k-1
+    const SolverState *st = A.ctl.st;
+    if (st->done) return;
+    const int rc = blockIdx.x % A.n_rt;
+    const int ct = blockIdx.x / A.n_rt;
+    const int N = A.n_rows, M = A.n_cols;
+    const int r0 = rc * R;
+    const int r1 = min(r0 + R, N);
+    const bool explicit_phi = A.ctl.k == 0;
+    float h1[RCH][VEC];
+    int gid[RCH][VEC];
+    split_rows<VEC, RCH>(A, explicit_phi, r0, h1, gid);
+    const int g_first = A.row_group[r0];
+    const int g_last = A.row_group[r1 - 1];
+    const float gamma = explicit_phi ? 0.0f : st->gamma;
+    const float fM = (float)M, fN = (float)N;
+    const double lam = (double)A.lambda;
+    float f1[RCH][VEC];
+    double racc[RCH][VEC];
+#pragma unroll
+    for (int r = 0; r < RCH; r++) {
+#pragma unroll
+        for (int e = 0; e < VEC; e++) {
+            f1[r][e] = explicit_phi ? h1[r][e] : (h1[r][e] + gamma) / fM;
+            gid[r][e] -= g_first;  // local group index within the chunk
+            racc[r][e] = 0.0;
+        }
+    }
+    const int ng = g_last - g_first + 1;
+    double oacc = 0.0;
+    const int j0 = ct * A.tc;
+    const int j1 = min(j0 + A.tc, M);
+    for (int j = j0; j < j1; j++) {
+        const float h2 = explicit_phi ? A.phi2[j] : A.bhat[j];
+        const float f2 = explicit_phi ? h2 : (h2 + gamma) / fN;
+        float y[RCH][VEC], cv[RCH][VEC];
+        split_prox<VEC, RCH>(A, j, r0, f2, f1, y, cv);
+        // group scales from the segments' partial norms, in chunk order
+        for (int gl = threadIdx.x; gl < ng; gl += kBlock) {
+            const int g = g_first + gl;
+            double n2 = 0.0;
+            for (int sg = A.gseg[g]; sg < A.gseg[g + 1]; sg++) n2 += A.pnorm[(size_t)sg * M + j];
+            s_scale[gl] = group_scale(n2, lam);
+        }
+        __syncthreads();
+        double csum = 0.0;
+#pragma unroll
+        for (int r = 0; r < RCH; r++) {
+            const int i0 = r0 + (r * kBlock + (int)threadIdx.x) * VEC;
+            if (i0 < N) {
+                float xo[VEC];
+#pragma unroll
+                for (int e = 0; e < VEC; e++) {
+                    const float v = y[r][e] * s_scale[gid[r][e]];
+                    xo[e] = v;
+                    racc[r][e] += (double)v;
+                    csum += (double)v;
+                    oacc += (double)v * (double)cv[r][e];
+                }
+                VecIO<VEC>::store(A.x + (size_t)j * N + i0, xo);
+            }
+        }
+        csum = block_sum(csum, s_red);  // its barriers also order s_scale's reuse
+        if (threadIdx.x == 0) A.pcol[(size_t)rc * M + j] = csum;
+    }
+#pragma unroll
+    for (int r = 0; r < RCH; r++) {
+#pragma unroll
+        for (int e = 0; e < VEC; e++) {
+            const int i = r0 + (r * kBlock + (int)threadIdx.x) * VEC + e;
+            if (i < N) A.prow[(size_t)ct * N + i] = racc[r][e];
+        }
+    }
+    const double o = block_sum(oacc, s_red);
+    if (threadIdx.x == 0) A.pobj[blockIdx.x] = o;
+}
+
 // ----------------------------------------------------------------- finalize
 
 template <int BATCH>
@@ -608,6 +829,24 @@ bool gl_tile_supported(const GlTile &t) {
 
 hipError_t launch_gl_pass(const GlTile &t, int grid, size_t lds, const GlPassArgs &a,
                           hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    if (t.split) {
+        // two launches; timing events bracket both (they include the dispatches)
+        if (e0 && e1) {
+            hipError_t er = hipEventRecord(e0, s);
+            if (er != hipSuccess) return er;
+        }
+#define DROT_GLS_CASE(V, R)                                                                \
+        if (t.vec == V && t.rch == R) {                                                    \
+            hipLaunchKernelGGL((gl_split_norm_kernel<V, R>), dim3(grid), dim3(kBlock), 0, s, a); \
+            hipLaunchKernelGGL((gl_split_apply_kernel<V, R>), dim3(grid), dim3(kBlock), 0, s, a); \
+        } else
+        DROT_GLS_CASE(4, 1) DROT_GLS_CASE(4, 4) DROT_GLS_CASE(1, 4) DROT_GLS_CASE(1, 16)
+        return hipErrorInvalidValue;
+#undef DROT_GLS_CASE
+        hipError_t er = hipGetLastError();
+        if (er == hipSuccess && e0 && e1) er = hipEventRecord(e1, s);
+        return er;
+    }
     if (t.wave) {
 #define DROT_GLW_CASE(V, R)                                                               \
         if (t.vec == V && t.rch == R) {                                                   \

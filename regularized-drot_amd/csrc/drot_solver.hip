@@ -66,6 +66,10 @@ struct Plan {
     size_t off_state = 0, off_phi1 = 0, off_phi2 = 0, off_a = 0, off_b = 0;
     size_t off_ahat = 0, off_bhat = 0, off_prow = 0, off_pcol = 0, off_pobj = 0;
     size_t off_pfin = 0, off_bounds = 0, off_rowgroup = 0, total = 0;
+    // GL long-column path: partial group norms [n_seg][n_cols] (fp64) and the
+    // per-group segment tables
+    size_t off_pnorm = 0, off_gseg = 0, off_gclo = 0;
+    int n_seg = 0;
     // QR zero-segment flags: two buffers of zf_bytes ([n_rt][n_cols]) at
     // off_zf, or none (zf_bytes == 0)
     size_t off_zf = 0, zf_bytes = 0;
@@ -108,7 +112,53 @@ void layout(Plan &P) {
     P.off_bounds = take((size_t)(P.n_groups + 1) * 4);
     P.off_rowgroup = take(P.kind == kGL ? N * 4 : 0);
     P.off_zf = take(2 * P.zf_bytes);
+    const bool split = P.kind == kGL && P.gt.split;
+    P.off_pnorm = take(split ? (size_t)P.n_seg * M * 8 : 0);
+    P.off_gseg = take(split ? (size_t)(P.n_groups + 1) * 4 : 0);
+    P.off_gclo = take(split ? (size_t)P.n_groups * 4 : 0);
     P.total = off;
+}
+
+// Group row bounds, core/kernel_glr.hpp:79-80 (groups <= 256 rows) and
+// :185-186 (larger groups), evaluated in fp32 as there.
+std::vector<int> gl_bounds(int n_rows, int n_groups) {
+    std::vector<int> hb((size_t)n_groups + 1);
+    const bool small = ceil_div(n_rows, n_groups) <= 256;
+    const double off = small ? 0.49 : 0.49999;
+    const float per = float(n_rows) / float(n_groups);
+    for (int g = 0; g <= n_groups; g++) hb[(size_t)g] = (int)((double)(per * float(g)) + off);
+    return hb;
+}
+
+// Segment tables of the long-column GL path for row chunks of `chunk` rows:
+// group g touches chunks gclo[g] .. and owns the partial-norm rows
+// gseg[g] .. gseg[g+1]-1, one per chunk (at least one, so an empty group's
+// slot stays inside its own range).
+// Returns the largest number of groups one chunk touches (the apply kernel
+// keeps one scale per such group in LDS, at most `chunk` of them).
+int gl_segments(const std::vector<int> &hb, int chunk, std::vector<int> *gseg,
+                std::vector<int> *gclo) {
+    const int G = (int)hb.size() - 1;
+    gseg->assign((size_t)G + 1, 0);
+    gclo->assign((size_t)G, 0);
+    for (int g = 0; g < G; g++) {
+        const int lo = hb[(size_t)g], hi = hb[(size_t)g + 1];
+        const int clo = lo / chunk;
+        const int chi = hi > lo ? (hi - 1) / chunk : clo;
+        (*gclo)[(size_t)g] = clo;
+        (*gseg)[(size_t)g + 1] = (*gseg)[(size_t)g] + (chi - clo + 1);
+    }
+    // groups of chunk c: from the group of its first row to that of its last
+    const int n_rows = hb.back();
+    int most = 0, g = 0;
+    for (int r0 = 0; r0 < n_rows; r0 += chunk) {
+        const int r1 = std::min(r0 + chunk, n_rows);
+        while (g + 1 < G && hb[(size_t)g + 1] <= r0) g++;
+        int g_last = g;
+        while (g_last + 1 < G && hb[(size_t)g_last + 1] <= r1 - 1) g_last++;
+        most = std::max(most, g_last - g + 1);
+    }
+    return most;
 }
 
 // Matrix size (elements) from which QR solves use zero-segment flags.  Time
@@ -285,17 +335,40 @@ bool plan_gl(int n_rows, int n_cols, int n_groups, bool vec4_ok, Plan *out,
     const int maxr = v4 ? 8 : 32;
     int rch = 1;
     while (rch < maxr && kBlock * P.gt.vec * rch < n_rows) rch *= 2;
-    if (kBlock * P.gt.vec * rch < n_rows) {
-        *why = "group-lasso solver supports n_rows <= 8192 (got " + std::to_string(n_rows) + ")";
-        return false;
+    const size_t block_lds = (size_t)n_rows * 8 + (size_t)n_groups * 4 + 16;
+    // DROT_GL_SPLIT=1 forces the long-column path with 1024-row chunks (tests
+    // compare it with the fused kernels on columns of a few chunks)
+    const char *split_env = getenv("DROT_GL_SPLIT");
+    const bool force_split = split_env && split_env[0] == '1';
+    if (force_split || kBlock * P.gt.vec * rch < n_rows || block_lds > kGlLdsLimit) {
+        // columns too long for one block: row chunks over blocks, group norms
+        // completed between two launches (gl_split_*_kernel)
+        P.gt.split = 1;
+        P.gt.wave = 0;
+        P.gt.rch = force_split ? (v4 ? 1 : 4) : (v4 ? 4 : 16);
+        const int chunk = kBlock * P.gt.vec * P.gt.rch;
+        P.n_rt = ceil_div(n_rows, chunk);
+        // about 1024 blocks over (chunk, column tile) pairs
+        P.tc = std::max(1, ceil_div(P.n_rt * n_cols, 1024));
+        P.n_ct = ceil_div(n_cols, P.tc);
+        P.grid_pass = P.n_rt * P.n_ct;
+        P.n_obj = P.grid_pass;
+        P.lane_per_group = (n_rows / std::max(1, n_groups)) < 64 ? 1 : 0;
+        P.rb = ceil_div(n_rows, kFinRows);
+        P.cb = ceil_div(n_cols, kFinRows);
+        std::vector<int> gseg, gclo;
+        if (gl_segments(gl_bounds(n_rows, n_groups), chunk, &gseg, &gclo) > chunk) {
+            *why = "group bounds put more groups than rows into one row chunk";
+            return false;
+        }
+        P.n_seg = gseg.back();
+        P.lds = 0;
+        layout(P);
+        *out = P;
+        return true;
     }
     P.gt.rch = rch;
-    P.lds = (size_t)n_rows * 8 + (size_t)n_groups * 4 + 16;
-    if (P.lds > kGlLdsLimit) {
-        *why = "group-lasso LDS tile exceeds 64 KiB for n_rows=" + std::to_string(n_rows) +
-               ", n_groups=" + std::to_string(n_groups);
-        return false;
-    }
+    P.lds = block_lds;
     // long columns: about 512 blocks (two per CU) hide more of each column's
     // load-and-reduce latency than 256, at the cost of twice the row partials
     // for the finalize (3000x3000: 44.4 -> 34.8 us/iteration; for columns
@@ -492,15 +565,9 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
         qa.kappa = in.kappa;
         qa.ctl = ctl;
     } else {
-        // group row bounds, core/kernel_glr.hpp:79-80 (groups <= 256 rows) and
-        // :185-186 (larger groups), evaluated in fp32 as there
         const int G = P.n_groups;
-        std::vector<int> hb(G + 1);
-        const bool small = ceil_div(P.n_rows, G) <= 256;
-        const double off = small ? 0.49 : 0.49999;
-        const float per = float(P.n_rows) / float(G);
-        for (int g = 0; g <= G; g++) hb[g] = (int)((double)(per * float(g)) + off);
-        if (hb[0] != 0 || hb[G] != P.n_rows)
+        const std::vector<int> hb = gl_bounds(P.n_rows, G);
+        if (hb[0] != 0 || hb[(size_t)G] != P.n_rows)
             return fail(DROT_ERR_UNSUPPORTED, "group bounds do not cover all rows");
         std::vector<int> rg((size_t)P.n_rows);  // group of each row
         for (int g = 0; g < G; g++)
@@ -509,6 +576,20 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
                                       hipMemcpyHostToDevice, stream));
         DROT_HIP_CHECK(hipMemcpyAsync(d.row_group, rg.data(), sizeof(int) * rg.size(),
                                       hipMemcpyHostToDevice, stream));
+        std::vector<int> gseg, gclo;
+        if (P.gt.split) {
+            gl_segments(hb, kBlock * P.gt.vec * P.gt.rch, &gseg, &gclo);
+            if (gseg.back() != P.n_seg)
+                return fail(DROT_ERR_UNSUPPORTED, "group segment count changed");
+            char *base = static_cast<char *>(ws);
+            ga.pnorm = reinterpret_cast<double *>(base + P.off_pnorm);
+            ga.gseg = reinterpret_cast<const int *>(base + P.off_gseg);
+            ga.gclo = reinterpret_cast<const int *>(base + P.off_gclo);
+            DROT_HIP_CHECK(hipMemcpyAsync(base + P.off_gseg, gseg.data(), sizeof(int) * gseg.size(),
+                                          hipMemcpyHostToDevice, stream));
+            DROT_HIP_CHECK(hipMemcpyAsync(base + P.off_gclo, gclo.data(), sizeof(int) * gclo.size(),
+                                          hipMemcpyHostToDevice, stream));
+        }
         DROT_HIP_CHECK(hipStreamSynchronize(stream));
         ga.x = in.x;
         ga.c = in.c;
@@ -525,6 +606,7 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
         ga.n_cols = P.n_cols;
         ga.n_groups = P.n_groups;
         ga.n_ct = P.n_ct;
+        ga.n_rt = P.n_rt;
         ga.tc = P.tc;
         ga.lane_per_group = P.lane_per_group;
         ga.step = in.step;
@@ -632,7 +714,7 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
                 DROT_HIP_CHECK(launch_qr_pass(P.qt, (k % 2) == 0, P.grid_pass, qa, stream, e0, e1));
             } else {
                 ga.ctl.k = k;
-                ga.ctl.tspan = span_ptr(k);
+                ga.ctl.tspan = P.gt.split ? nullptr : span_ptr(k);
                 DROT_HIP_CHECK(launch_gl_pass(P.gt, P.grid_pass, P.lds, ga, stream, e0, e1));
             }
             if (k < in.max_iters) {

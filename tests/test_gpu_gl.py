@@ -1,7 +1,9 @@
 """GPU parity of GL-DROT against the CPU oracle, through the C ABI (libdrot.so).
 
 Bar: max|X_gpu - X_oracle| <= 1e-5 * max|X_oracle| after the same number of
-iterations, identical n_iter for full solves.
+iterations, identical n_iter for full solves.  Covers the wave-per-column and
+block-per-column kernels and the long-column path (any n_rows: row chunks over
+blocks, group norms completed between two launches).
 """
 import numpy as np
 import pytest
@@ -109,3 +111,90 @@ def test_gl_full_solve_ragged_within_conditioning(device):
     assert r.n_iter == ref.n_iter
     assert rel_err(x, ref.x) <= 2 * sens
     assert abs(r.objective_f64 - ref.objective) <= 2 * sens_obj
+
+
+# Long columns: row chunks over blocks with the group norms completed between
+# two launches (any n_rows; the fused kernels hold a column in one block).
+# 12001 rows run the one-float kernels; 3000 groups of 3 rows sum per lane.
+LONG = [(12000, 1000, 4), (10000, 500, 2), (10000, 500, 40), (12001, 300, 3),
+        (9000, 100, 3000), (8200, 64, 2)]
+
+
+@pytest.mark.parametrize("n_rows,n_cols,n_groups", LONG)
+@pytest.mark.parametrize("iters", [1, 5, 30])
+def test_gl_long_columns_trajectory(device, n_rows, n_cols, n_groups, iters):
+    c = problems.reference_test_cost(n_rows, n_cols, seed=n_rows + 7 * n_cols)
+    p, q = problems.uniform_marginals(n_rows, n_cols)
+    from reg_drot import _abi as abi
+    prm = abi.glr_params_from_rho(2.0, 1e-2, n_rows, n_cols, n_groups, iters, 0.0)
+    print(f"[gl long] {n_rows}x{n_cols} G={n_groups} iters={iters}", flush=True)
+    x, r, phi1, phi2, alpha = gpu_gl(c, p, q, prm, device)
+    ref = O.gl_solve(c, p, q, n_groups, np.float32(prm.step_size), np.float32(prm.lambda_),
+                     iters, 0.0)
+    assert r.iterations_run == iters == ref.n_iter
+    assert rel_err(x, ref.x) <= PLAN_RTOL
+    assert rel_err(phi1, ref.state.phi1) <= PLAN_RTOL
+    assert rel_err(phi2, ref.state.phi2) <= PLAN_RTOL
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("n_rows,n_cols,n_groups", [(12000, 1000, 4), (10000, 500, 40)])
+def test_gl_long_columns_full_solve(device, n_rows, n_cols, n_groups):
+    c = problems.cloud_cost(n_rows, n_cols, dim=16, seed=n_groups)
+    p, q = problems.uniform_marginals(n_rows, n_cols)
+    from reg_drot import _abi as abi
+    prm = abi.glr_params_from_rho(2.0, 1e-3, n_rows, n_cols, n_groups, 20000, 1e-4)
+    print(f"[gl long solve] {n_rows}x{n_cols} G={n_groups}", flush=True)
+    x, r, *_ = gpu_gl(c, p, q, prm, device)
+    ref = O.gl_solve(c, p, q, n_groups, np.float32(prm.step_size), np.float32(prm.lambda_),
+                     20000, np.float32(1e-4))
+    print(f"[gl long solve] n_iter {r.n_iter} vs {ref.n_iter}, rel {rel_err(x, ref.x):.2e}",
+          flush=True)
+    assert r.converged == int(ref.converged) == 1
+    assert r.n_iter == ref.n_iter
+    assert rel_err(x, ref.x) <= PLAN_RTOL
+    assert abs(r.objective_f64 - ref.objective) <= 1e-6 * abs(ref.objective)
+
+
+@pytest.mark.parametrize("n_rows,n_cols,n_groups", [(1500, 1000, 2), (3000, 300, 4),
+                                                    (2999, 300, 7), (600, 200, 600),
+                                                    (1000, 500, 5), (5000, 64, 1)])
+@pytest.mark.parametrize("iters", [1, 6, 31])
+def test_gl_split_path_on_short_columns(device, monkeypatch, n_rows, n_cols, n_groups, iters):
+    """DROT_GL_SPLIT=1 runs the long-column kernels with 1024-row chunks, so
+    groups span several chunks on columns the oracle steps quickly."""
+    monkeypatch.setenv("DROT_GL_SPLIT", "1")
+    c = problems.reference_test_cost(n_rows, n_cols, seed=3 * n_rows + n_cols)
+    p, q = problems.uniform_marginals(n_rows, n_cols)
+    from reg_drot import _abi as abi
+    prm = abi.glr_params_from_rho(2.0, 1e-2, n_rows, n_cols, n_groups, iters, 0.0)
+    print(f"[gl split] {n_rows}x{n_cols} G={n_groups} iters={iters}", flush=True)
+    x, r, phi1, phi2, alpha = gpu_gl(c, p, q, prm, device)
+    ref = O.gl_solve(c, p, q, n_groups, np.float32(prm.step_size), np.float32(prm.lambda_),
+                     iters, 0.0)
+    assert r.iterations_run == iters == ref.n_iter
+    assert rel_err(x, ref.x) <= PLAN_RTOL
+    assert rel_err(phi1, ref.state.phi1) <= PLAN_RTOL
+
+
+def test_gl_split_path_full_solve_stops_like_the_oracle(device, monkeypatch):
+    """The stopping test (every iteration, residual <= eps) and the finish of
+    the norm launch's prologue carry over to the apply launch.  The problem is
+    the domain-adaptation construction with the core lambda, whose converged
+    plan moves by only 1.4e-6 when the cost changes by one ulp (uniform
+    random costs with the PyTorch lambda move GL plans of this size by 1e-4 to
+    1e-3, more than any summation order can match)."""
+    monkeypatch.setenv("DROT_GL_SPLIT", "1")
+    n_rows, n_cols, n_groups = 2100, 700, 2
+    c, _ = problems.domain_adaptation(n_rows, n_cols, seed=3)
+    p, q = problems.uniform_marginals(n_rows, n_cols)
+    from reg_drot import _abi as abi
+    for iters, eps in ((20000, 1e-4), (37, 0.0)):
+        prm = abi.glr_params_from_step(2.0 / (n_rows + n_cols), 1e-3, n_rows, n_cols, n_groups,
+                                       iters, eps)
+        x, r, *_ = gpu_gl(c, p, q, prm, device)
+        ref = O.gl_solve(c, p, q, n_groups, np.float32(prm.step_size),
+                         np.float32(prm.lambda_), iters, np.float32(eps))
+        assert r.converged == int(ref.converged)
+        assert r.n_iter == ref.n_iter
+        assert rel_err(x, ref.x) <= PLAN_RTOL
