@@ -67,6 +67,10 @@ def _load():
         fn.argtypes = [i, i, i, _i32p, rp, rp, rp, rt, rt, i, rt, i, rp, rp, rp,
                        rp, rp, ctypes.POINTER(rt), ctypes.POINTER(i), _f64p, i]
         fn.restype = i
+    for sfx, rp, rt in (("_f32", _f32p, f), ("_f64", _f64p, d)):
+        fn = getattr(lib, "npdrot_qr_solve" + sfx)
+        fn.argtypes = [i, i, rp, rp, rp, rt, rt, i, d, rp, ctypes.POINTER(d), i]
+        fn.restype = i
     lib.oracle_qr_params_torch.argtypes = [f, f, i, i, ctypes.POINTER(f),
                                            ctypes.POINTER(f)]
     lib.oracle_qr_params_torch.restype = None
@@ -273,3 +277,24 @@ def gl_solve(c, p, q, n_groups, step, lam, max_iters, eps, warm=True,
         raise MemoryError("oracle_gl_solve failed")
     st.alpha = al.value
     return SolveResult(st.x, nit.value, out[0], out[1], bool(out[2]), st)
+
+
+# ---------------------------------------------------------------- npdrot, OpenMP
+
+def npdrot_solve(y0, c, p, q, step, reg, max_iters, eps, dtype=np.float64, nthreads=None):
+    """The reference's CPU solver (npdrot regdrot + quadratic prox) restated in
+    C with OpenMP, pass for pass (npdrot_omp.c).  y0, c: (n_cols, n_rows)
+    arrays in the solver layout (= npdrot's (m, n) F-order arrays); p: (n_rows,),
+    q: (n_cols,).  Returns (X, iterations, last primal residual): X is the
+    final prox output, as npdrot returns it."""
+    c = np.ascontiguousarray(c, dtype)
+    n_cols, n_rows = c.shape
+    x = np.array(y0, dtype=dtype, order="C", copy=True)
+    r = ctypes.c_double()
+    k = getattr(_load(), "npdrot_qr_solve" + _sfx(dtype))(
+        n_rows, n_cols, c, np.ascontiguousarray(p, dtype), np.ascontiguousarray(q, dtype),
+        step, reg, int(max_iters), float(eps), x, ctypes.byref(r),
+        nthreads or default_threads())
+    if k < 0:
+        raise MemoryError("npdrot_qr_solve failed")
+    return x, k, r.value

@@ -105,3 +105,48 @@ def test_quad_prox_value():
     want = np.maximum(np.array([[0.0, -1.0], [1.75, 0.125]]), 0.0) / 2.0
     assert np.array_equal(y, want)
     assert val == pytest.approx(0.5 * 2.0 * (want ** 2).sum())
+
+
+@pytest.mark.parametrize("n_rows,n_cols,rw", [(64, 48, 1e-2), (300, 200, 1e-3), (129, 77, 1e-1)])
+@pytest.mark.parametrize("iters", [1, 2, 40, 400])
+def test_openmp_restatement_matches_numpy(n_rows, n_cols, rw, iters):
+    """oracle/npdrot_omp.c (the CPU baseline: npdrot's passes in C with
+    OpenMP) against the numpy restatement, fp64: same plan to 1e-12 after the
+    same iterations, same residual sequence end; the fp32 instance follows the
+    fp64 one to fp32 accuracy."""
+    c = _cost(n_rows, n_cols, 3 * n_rows + n_cols)
+    p = np.full(n_rows, 1.0 / n_rows)
+    q = np.full(n_cols, 1.0 / n_cols)
+    s, k = (float(v) for v in O.qr_params_torch(2.0, rw, n_rows, n_cols))
+    reg = (k - 1.0) / s
+    xn, info = N.regdrot(np.full((n_rows, n_cols), s), c.T, p, q, reg, max_iters=iters,
+                         eps_abs=-1.0, step=s)
+    y0 = np.full((n_cols, n_rows), s)  # solver layout (n_cols, n_rows)
+    xc, kc, rc = O.npdrot_solve(y0, c, p, q, s, reg, iters, -1.0, dtype=np.float64, nthreads=4)
+    assert kc == info["num_iters"] == iters
+    assert np.abs(xc - xn.T).max() <= 1e-12 * np.abs(xn).max()
+    assert abs(rc - info["r_primal"][-1]) <= 1e-9 * info["r_primal"][-1] + 1e-15
+    x32, k32, _ = O.npdrot_solve(y0, c, p, q, s, reg, iters, -1.0, dtype=np.float32,
+                                 nthreads=3)
+    assert k32 == iters
+    assert np.abs(x32 - xc).max() <= 1e-4 * np.abs(xc).max()
+
+
+def test_openmp_restatement_stops_on_the_residual():
+    """The stop rule of solver.py:165-166 (r_primal <= eps_abs) and the extra
+    prox after the loop, against the numpy restatement; thread counts do not
+    change the result beyond rounding."""
+    n_rows, n_cols = 256, 128
+    c = _cost(n_rows, n_cols, 5)
+    p = np.full(n_rows, 1.0 / n_rows)
+    q = np.full(n_cols, 1.0 / n_cols)
+    s, k = (float(v) for v in O.qr_params_torch(2.0, 1e-2, n_rows, n_cols))
+    reg = (k - 1.0) / s
+    xn, info = N.regdrot(np.full((n_rows, n_cols), s), c.T, p, q, reg, max_iters=20000,
+                         eps_abs=1e-6, step=s)
+    y0 = np.full((n_cols, n_rows), s)
+    for nt in (1, 5):
+        xc, kc, rc = O.npdrot_solve(y0, c, p, q, s, reg, 20000, 1e-6, nthreads=nt)
+        assert kc == info["num_iters"]
+        assert rc <= 1e-6
+        assert np.abs(xc - xn.T).max() <= 1e-11 * np.abs(xn).max()
