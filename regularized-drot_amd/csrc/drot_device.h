@@ -76,12 +76,14 @@ __device__ __forceinline__ void record_span(unsigned long long *tspan, unsigned 
 
 // ---------------------------------------------------------------- reductions
 
-// A double moved across lanes by a DPP control (two 32-bit DPP moves).
+// A double moved across lanes by a DPP control (two 32-bit DPP moves).  Only
+// controls that give every lane a source (row rotations, quad permutations)
+// are used, so no lane keeps an old value and none needs initialising.
 template <int CTRL>
 __device__ __forceinline__ double dpp_move(double v) {
     const long long u = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)u, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), CTRL, 0xF, 0xF, true);
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
 }
 

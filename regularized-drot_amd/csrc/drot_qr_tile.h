@@ -146,11 +146,330 @@ __device__ __forceinline__ unsigned long long group_bits(unsigned long long b, i
     }
 }
 
-// ZS: zero-segment flags in use (J.zin and J.zout set); without them the
-// loads are the plain ones below.
+// End of a tile: the row partials of the tile's columns (fixed-order combine of
+// the kWavesPerBlock*CPW column slots) and, on even passes, the objective
+// partial.  The objective's wave sums meet behind the same barrier (in their
+// own LDS slots: s_red may still be read by the prologue of a slower wave) and
+// are added in wave order as block_sum does.
+template <int VEC, int LPC, bool EVEN>
+__device__ __forceinline__ void tile_partials_out(const QrTileJob &J, const double (&racc)[VEC],
+                                                  double oacc, double (*s_row)[LPC * VEC]) {
+    constexpr int CPW = kWave / LPC;
+    constexpr int TR = LPC * VEC;
+    __shared__ double s_obj[kWavesPerBlock];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = wave_uniform(threadIdx.x / kWave);
+    const int slot = w * CPW + lane / LPC;
+    const int rl = lane % LPC;
+#pragma unroll
+    for (int e = 0; e < VEC; e++) s_row[slot][rl * VEC + e] = racc[e];
+    if (EVEN) {
+        const double ow = wave_sum(oacc);
+        if (lane == 0) s_obj[w] = ow;
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < TR; r += kBlock) {
+        double sum = 0.0;
+#pragma unroll
+        for (int q = 0; q < kWavesPerBlock * CPW; q++) sum += s_row[q][r];
+        const int i = J.rt * TR + r;
+        if (i < J.n_rows) J.prow[(size_t)J.ct * J.n_rows + i] = sum;
+    }
+    if (EVEN && threadIdx.x == 0) {
+        double o = 0.0;
+#pragma unroll
+        for (int q = 0; q < kWavesPerBlock; q++) o += s_obj[q];
+        *J.pobj = o;
+    }
+}
+
+// ------------------------------------------------------------ lean tile loop
+//
+// When every VEC-row chunk lies inside one column (ld % VEC == 0) and a tile's
+// columns span less than 2^31 bytes, a tile runs without a branch per element
+// or per lane.  X, C, the column potentials, the column partials and the zero
+// flags are accessed through buffer resources over the tile's own columns; a
+// chunk outside the problem (a row past n_rows, a column past n_cols) or a
+// zero-flagged X gets an offset past the resource's range, so the hardware
+// returns zeros for its loads and drops its stores.  Rows past n_rows and
+// columns past n_cols get the potential -inf, which makes their X' +0, so they
+// add +0 to every partial sum: the sums, the flags and the stored X keep the
+// bits of the masked formulation of qr_tile_generic.  Padding rows of a batch
+// column (n_rows <= row < ld) read C as 0 on even passes, so whatever the
+// padding holds, the objective partial gets +0 from them and they store +0.
+// The batches of loads alternate between two register sets, so no copy
+// rotates the software pipeline.
+
+// Byte offset past every tile's range (tiles span less than 2^31 bytes).
+constexpr unsigned kOob = 0x80000000u;
+
+// The largest number of columns a batch of loads reaches past the tile
+// (U * kWavesPerBlock * CPW <= 4 * 4 * 4).
+constexpr int kLeanColSlack = 64;
+
+template <int VEC>
+__device__ __forceinline__ bool qr_lean_ok(const QrTileJob &J) {
+    return (J.ld % VEC) == 0 &&
+           (unsigned long long)(J.tc + kLeanColSlack) * (unsigned long long)J.ld * 4ull <
+               (1ull << 31);
+}
+
+template <int VEC>
+struct BufIo;
+
+template <>
+struct BufIo<4> {
+    static __device__ __forceinline__ void load(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                                float (&v)[4]) {
+        const u32x4 d = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+        v[0] = __uint_as_float(d.x);
+        v[1] = __uint_as_float(d.y);
+        v[2] = __uint_as_float(d.z);
+        v[3] = __uint_as_float(d.w);
+    }
+    template <int POL>
+    static __device__ __forceinline__ void store(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                                 const float (&v)[4]) {
+        u32x4 d;
+        d.x = __float_as_uint(v[0]);
+        d.y = __float_as_uint(v[1]);
+        d.z = __float_as_uint(v[2]);
+        d.w = __float_as_uint(v[3]);
+        __builtin_amdgcn_raw_buffer_store_b128(d, r, (int)off, 0, POL);
+    }
+};
+
+template <>
+struct BufIo<1> {
+    static __device__ __forceinline__ void load(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                                float (&v)[1]) {
+        v[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+    }
+    template <int POL>
+    static __device__ __forceinline__ void store(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                                 const float (&v)[1]) {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v[0]), r, (int)off, 0, POL);
+    }
+};
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_over(const void *p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, kBufDword3);
+}
+
+// div_kappa's quotient a/b (any sign of a) leaves the range where its
+// correction step is exact
+__device__ __forceinline__ bool div_rn_slow(float a, float q) {
+    return a != 0.0f && !(fabsf(q) >= 0x1p-90f && fabsf(q) <= 0x1p+120f);
+}
+
 template <int VEC, int LPC, bool EVEN, int ST, int U, bool ZS>
-__device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, bool leader_block,
-                                        double (*s_row)[LPC * VEC], double *s_red) {
+__device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &ctl,
+                                             bool leader_block, double (*s_row)[LPC * VEC],
+                                             double *s_red) {
+    constexpr int CPW = kWave / LPC;
+    constexpr int TR = LPC * VEC;
+    constexpr int kColStride = kWavesPerBlock * CPW;
+    constexpr int kPol = ST == 2 ? kPolicySc1 : 0;
+    static_assert(U * kColStride <= kLeanColSlack, "batch reaches past the column slack");
+    const int N = J.n_rows, M = J.n_cols, ld = J.ld;
+    const int nstep = J.nstep;
+    const float fM = (float)M, fN = (float)N;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = wave_uniform(threadIdx.x / kWave);
+    const int cs = lane / LPC;
+    const int rl = lane % LPC;
+    const int row = J.rt * TR + rl * VEC;
+    const bool row_ok = row < N;
+    const int tile_col0 = J.ct * J.tc;
+    const int tile_cols = min(J.tc, M - tile_col0);
+    const int jt0 = w * CPW + cs;  // tile column of step t: jt0 + t * kColStride
+    const bool explicit_phi = (ctl.k == 0);
+    const float *hat2 = explicit_phi ? J.phi2 : J.bhat;
+    // the tile holds rows past n_rows (padding of a batch column, or none at
+    // all); a scalar int, so that its branch is a scalar one
+    const int pad = __builtin_amdgcn_readfirstlane((J.rt + 1) * TR > N ? 1 : 0);
+
+    const size_t col_base = (size_t)tile_col0 * (size_t)ld;
+    const unsigned tile_bytes = (unsigned)tile_cols * (unsigned)ld * 4u;
+    const __amdgpu_buffer_rsrc_t xr = buffer_over(J.x + col_base, tile_bytes);
+    const __amdgpu_buffer_rsrc_t cr = buffer_over(J.c + col_base, tile_bytes);
+    const __amdgpu_buffer_rsrc_t hr = buffer_over(hat2 + tile_col0, (unsigned)tile_cols * 4u);
+    const size_t part_base = (size_t)J.rt * M + tile_col0;
+    const __amdgpu_buffer_rsrc_t pr = buffer_over(J.pcol + part_base, (unsigned)tile_cols * 8u);
+    const __amdgpu_buffer_rsrc_t zr =
+        ZS ? buffer_over(J.zout + part_base, (unsigned)tile_cols) : pr;
+    const unsigned x_lane = row_ok ? ((unsigned)jt0 * (unsigned)ld + (unsigned)row) * 4u : kOob;
+    const unsigned x_step = (unsigned)kColStride * (unsigned)ld * 4u;
+    const unsigned h_lane = (unsigned)jt0 * 4u;
+    const unsigned p_lane = rl == 0 ? (unsigned)jt0 * 8u : kOob;
+    const unsigned z_lane = rl == 0 ? (unsigned)jt0 : kOob;
+    const int col_lim = tile_cols - jt0;  // step t is a column of the problem iff t * kColStride < col_lim
+
+    // zero flags of this column group's segments from the previous pass, bit t
+    // for column step t (see qr_tile_generic)
+    using ZMask = std::conditional_t<(LPC > 32), unsigned long long, unsigned>;
+    constexpr int kZBits = 8 * (int)sizeof(ZMask);
+    ZMask zmask = 0;
+    if (ZS) {
+        const int jz = tile_col0 + jt0 + rl * kColStride;
+        const bool zb = rl < nstep && jz < M && J.zin[(size_t)J.rt * M + jz] != 0;
+        zmask = (ZMask)group_bits<LPC>(__builtin_amdgcn_ballot_w64(zb), cs);
+    }
+
+    float xa[U][VEC], ca[U][VEC], ha[U];
+    float xb[U][VEC], cb[U][VEC], hb[U];
+    // always inlined: the batches are register arrays (a call would put them in scratch)
+    auto load_batch = [&](int t0, float (&xv)[U][VEC], float (&cv)[U][VEC],
+                          float (&hv)[U]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u;  // steps past nstep read columns past the tile: zeros
+            const unsigned off = x_lane + (unsigned)t * x_step;
+            hv[u] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                hr, (int)(h_lane + (unsigned)(t * kColStride) * 4u), 0, 0));
+            if constexpr (ZS) {
+                const bool z = (zmask >> (t & (kZBits - 1))) & 1;
+                // a flagged segment: the even pass reads no X (+0), the odd
+                // pass reads C (and X as +0) to form what the even pass left
+                if (EVEN) BufIo<VEC>::load(cr, off, cv[u]);
+                BufIo<VEC>::load(xr, z ? kOob : off, xv[u]);
+                if (!EVEN) BufIo<VEC>::load(cr, z ? off : kOob, cv[u]);
+            } else {
+                BufIo<VEC>::load(xr, off, xv[u]);
+                if (EVEN) BufIo<VEC>::load(cr, off, cv[u]);
+            }
+        }
+    };
+    load_batch(0, xa, ca, ha);
+    float h1[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; e++)
+        h1[e] = row + e < N ? (explicit_phi ? J.phi1[row + e] : J.ahat[row + e]) : 0.0f;
+
+    const IterStart is = iteration_prologue(ctl, s_red, leader_block);
+    if (is.finish) return;
+    const float gamma = is.gamma;
+    const float s = J.step, kappa = J.kappa;
+    const float rkappa = 1.0f / kappa;
+    const float g2 = explicit_phi ? 0.0f : gamma;
+    const float d2 = explicit_phi ? 1.0f : fN;
+    const float rd2 = 1.0f / d2;
+    constexpr unsigned kQLo = 0x12800000u;  // bits of 2^-90
+    constexpr unsigned kQHi = 0x7B800000u;  // bits of 2^120
+    // conservative screen for div_kappa_slow: y in [2^-88 kappa, 2^118 kappa]
+    // gives a quotient in [2^-90, 2^120]; both bounds are exact for kappa in
+    // [2^-30, 2^8], and outside that every positive y goes to the exact test
+    const bool screen = kappa >= 0x1p-30f && kappa <= 0x1p+8f;
+    const unsigned ylo_m1 = screen ? __float_as_uint(0x1p-88f * kappa) - 1u : 0xFFFFFFFEu;
+    const unsigned yhi = screen ? __float_as_uint(0x1p+118f * kappa) : 0u;
+
+    float f1[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; e++)
+        f1[e] = row + e < N ? (explicit_phi ? h1[e] : (h1[e] + gamma) / fM) : -INFINITY;
+    double racc[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; e++) racc[e] = 0.0;
+    double oacc = 0.0;
+
+    auto run_batch = [&](int t0, float (&xv)[U][VEC], float (&cv)[U][VEC],
+                         float (&hv)[U]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u;
+            if (t >= nstep) break;  // wave-uniform
+            // phi2 = (hat + gamma) / n_rows; at k == 0 the array value itself,
+            // as (value + 0) / 1 (a zero's sign aside, which no X' sees)
+            const float num = hv[u] + g2;
+            float f2 = div_kappa(num, d2, rd2);
+            const unsigned qb = __float_as_uint(f2) & 0x7FFFFFFFu;
+            // (two ballots of plain compares: their lane masks meet in one
+            // scalar and, with no per-lane boolean to materialise)
+            if (__builtin_amdgcn_ballot_w64(num != 0.0f) &
+                __builtin_amdgcn_ballot_w64(qb - kQLo > kQHi - kQLo))
+                if (div_rn_slow(num, f2)) f2 = num / d2;
+            f2 = t * kColStride < col_lim ? f2 : -INFINITY;
+            if (EVEN && pad) {
+#pragma unroll
+                for (int e = 0; e < VEC; e++)
+                    if (row + e >= N) cv[u][e] = 0.0f;
+            }
+            float yv[VEC], xq[VEC];
+            unsigned lo = 0xFFFFFFFFu, hi = 0u;
+#pragma unroll
+            for (int e = 0; e < VEC; e++) {
+                // odd pass, flagged segment: X = +0, C read, giving fmaf(-s, c, +0);
+                // elsewhere C reads 0 and fmaf(-s, 0, x) == x
+                const float xe = (!EVEN && ZS) ? __builtin_fmaf(-s, cv[u][e], xv[u][e]) : xv[u][e];
+                const float tt = (xe + f1[e]) + f2;
+                const float y = EVEN ? __builtin_fmaf(-s, cv[u][e], tt) : tt;
+                yv[e] = fmaxf(y, 0.0f);
+                xq[e] = div_kappa(yv[e], kappa, rkappa);
+                const unsigned b = __float_as_uint(yv[e]);
+                lo = min(lo, b - 1u);  // +0 maps to the largest value
+                hi = max(hi, b);
+            }
+            if (__builtin_amdgcn_ballot_w64(lo < ylo_m1 || hi > yhi)) {
+#pragma unroll
+                for (int e = 0; e < VEC; e++)
+                    if (div_kappa_slow(yv[e], xq[e])) xq[e] = yv[e] / kappa;
+            }
+            float xo[VEC];
+            double csum = 0.0;
+            unsigned nzb = 0u;
+#pragma unroll
+            for (int e = 0; e < VEC; e++) {
+                const float xp = xq[e];  // +0 or positive: never -0
+                xo[e] = EVEN ? __builtin_fmaf(-s, cv[u][e], xp) : xp;
+                const double d = (double)xp;
+                racc[e] += d;
+                csum = e == 0 ? d : csum + d;
+                // the fp64 product of two floats is exact, so the fma rounds
+                // once, as the separate product and sum do
+                if (EVEN) oacc = __builtin_fma(d, (double)cv[u][e], oacc);
+                nzb |= __float_as_uint(xp);
+            }
+            const unsigned off = x_lane + (unsigned)t * x_step;
+            unsigned st_off = off;
+            if constexpr (ZS) {
+                const bool zseg =
+                    group_bits<LPC>(__builtin_amdgcn_ballot_w64(nzb != 0u), cs) == 0ull;
+                __builtin_amdgcn_raw_buffer_store_b8((unsigned char)(zseg ? 1 : 0), zr,
+                                                     (int)(z_lane + (unsigned)(t * kColStride)),
+                                                     0, 0);
+                st_off = zseg ? kOob : off;
+            }
+            BufIo<VEC>::template store<kPol>(xr, st_off, xo);
+            csum = seg_sum<LPC>(csum);
+            const unsigned long long cb64 = (unsigned long long)__double_as_longlong(csum);
+            typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+            u32x2 cw;
+            cw.x = (unsigned)cb64;
+            cw.y = (unsigned)(cb64 >> 32);
+            __builtin_amdgcn_raw_buffer_store_b64(
+                cw, pr, (int)(p_lane + (unsigned)(t * kColStride) * 8u), 0, 0);
+        }
+    };
+
+    for (int t0 = 0; t0 < nstep; t0 += 2 * U) {
+        if (J.prio) set_issue_priority(nstep - t0, nstep);
+        if (t0 + U < nstep) load_batch(t0 + U, xb, cb, hb);
+        run_batch(t0, xa, ca, ha);
+        if (t0 + U >= nstep) break;
+        if (J.prio) set_issue_priority(nstep - t0 - U, nstep);
+        if (t0 + 2 * U < nstep) load_batch(t0 + 2 * U, xa, ca, ha);
+        run_batch(t0 + U, xb, cb, hb);
+    }
+    tile_partials_out<VEC, LPC, EVEN>(J, racc, oacc, s_row);
+}
+
+// The masked formulation, for any leading dimension: chunks cut by the column
+// end are read and written element by element.  ZS: zero-segment flags in use
+// (J.zin and J.zout set); without them the loads are the plain ones below.
+template <int VEC, int LPC, bool EVEN, int ST, int U, bool ZS>
+__device__ __forceinline__ void qr_tile_generic(const QrTileJob &J, const IterCtl &ctl,
+                                                bool leader_block, double (*s_row)[LPC * VEC],
+                                                double *s_red) {
     constexpr int CPW = kWave / LPC;  // columns per wave step
     constexpr int TR = LPC * VEC;     // tile rows
     constexpr int kColStride = kWavesPerBlock * CPW;
@@ -351,32 +670,17 @@ __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, 
         }
     }
 
-    // strided-axis partials: fixed-order combine of the 4*CPW column slots;
-    // the objective's wave sums meet behind the same barrier (in their own LDS
-    // slots: s_red may still be read by the prologue of a slower wave), and
-    // are added in wave order as block_sum does
-    __shared__ double s_obj[kWavesPerBlock];
-    const int slot = w * CPW + cs;
-#pragma unroll
-    for (int e = 0; e < VEC; e++) s_row[slot][rl * VEC + e] = racc[e];
-    if (EVEN) {
-        const double ow = wave_sum(oacc);
-        if (lane == 0) s_obj[w] = ow;
+    tile_partials_out<VEC, LPC, EVEN>(J, racc, oacc, s_row);
+}
+
+template <int VEC, int LPC, bool EVEN, int ST, int U, bool ZS>
+__device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, bool leader_block,
+                                        double (*s_row)[LPC * VEC], double *s_red) {
+    if (qr_lean_ok<VEC>(J)) {  // wave-uniform
+        qr_tile_lean<VEC, LPC, EVEN, ST, U, ZS>(J, ctl, leader_block, s_row, s_red);
+        return;
     }
-    __syncthreads();
-    for (int r = threadIdx.x; r < TR; r += kBlock) {
-        double sum = 0.0;
-#pragma unroll
-        for (int q = 0; q < kWavesPerBlock * CPW; q++) sum += s_row[q][r];
-        const int i = J.rt * TR + r;
-        if (i < N) J.prow[(size_t)J.ct * N + i] = sum;
-    }
-    if (EVEN && threadIdx.x == 0) {
-        double o = 0.0;
-#pragma unroll
-        for (int q = 0; q < kWavesPerBlock; q++) o += s_obj[q];
-        *J.pobj = o;
-    }
+    qr_tile_generic<VEC, LPC, EVEN, ST, U, ZS>(J, ctl, leader_block, s_row, s_red);
 }
 
 }  // namespace drot
