@@ -263,10 +263,12 @@ drot_status drot_qr_batch_backward_f32(int n_problems, const drot_batch_problem 
 
 /* ---------------------------------------------------------------- tuning */
 
-/* Tile of the QR pass kernel: vec = floats per access (1 or 4), lanes_per_col
- * (16, 32, 64) lanes share a column, col_steps column steps per block,
- * store_policy 2 = write-through X stores, 0 = plain stores, unroll (2 or 4)
- * column steps loaded per batch.  Replaces the reference's launch-size
+/* Tile of the QR pass kernel: vec = rows of a column per lane (1 or 4: one
+ * access; 8: two 16-byte accesses, for n_rows a multiple of 4 and 16-byte
+ * aligned buffers), lanes_per_col lanes share a column (16, 32, 64; 8 or 16
+ * for vec 8), col_steps column steps per block, store_policy 2 = write-through
+ * X stores, 0 = plain stores, unroll column steps loaded per batch (2 or 4;
+ * 1 for vec 8).  Replaces the reference's launch-size
  * heuristic _q_get_work_size_update_x (core/drot_qr.hpp:236-239) and its
  * launch macros (core/param_qr.hpp:4-24). */
 typedef struct drot_qr_tile {

@@ -95,10 +95,11 @@ __global__ __launch_bounds__(kBlock) void batch_init_kernel(QrBatchArgs A) {
 // One pass over every listed tile; the list holds the tiles of the active
 // problems in row-tile-major order per problem, and blocks take contiguous
 // runs of it per XCD (see qr_pass_kernel).
-template <bool EVEN, bool ZS>
+template <bool EVEN, bool ZS, int VEC, int LPC>
 __global__ __launch_bounds__(kBlock) void batch_pass_kernel(QrBatchArgs A) {
-    constexpr int LPC = kBatchLpc;
     constexpr int CPW = kWave / LPC;
+    constexpr int kSteps = kBatchTileCols / (kWavesPerBlock * CPW);
+    static_assert(VEC * LPC == kBatchTileRows, "64-row batch tiles");
     __shared__ double s_row[kWavesPerBlock * CPW][kBatchTileRows];
     __shared__ double s_red[3 * kWavesPerBlock];
     const int per_xcd = (A.n_tiles + kXcds - 1) / kXcds;
@@ -123,9 +124,9 @@ __global__ __launch_bounds__(kBlock) void batch_pass_kernel(QrBatchArgs A) {
     J.ld = P.ld;
     J.rt = bt.rt;
     J.ct = bt.ct;
-    J.nstep = P.nstep;
+    J.nstep = kSteps;
     J.prio = 0;
-    J.tc = P.nstep * kWavesPerBlock * CPW;
+    J.tc = kBatchTileCols;
     J.step = P.step;
     J.kappa = P.kappa;
     J.zin = ZS ? A.zf + (size_t)((A.k + 1) & 1) * A.zf_len + P.zf_off : nullptr;
@@ -133,7 +134,8 @@ __global__ __launch_bounds__(kBlock) void batch_pass_kernel(QrBatchArgs A) {
     const IterCtl ctl = batch_ctl(A, P, prob);
     // plain X stores: a batch's footprint lives in HBM, and write-through stores
     // made the mixed 256-problem forward 5% slower (0.99 vs 0.94 s)
-    qr_tile<4, LPC, EVEN, 0, 2, ZS>(J, ctl, bt.rt == 0 && bt.ct == 0, s_row, s_red);
+    qr_tile<VEC, LPC, EVEN, 0, (VEC == 8 ? 1 : 2), ZS>(J, ctl, bt.rt == 0 && bt.ct == 0, s_row,
+                                                       s_red);
 }
 
 __global__ __launch_bounds__(kBlock) void batch_finalize_kernel(QrBatchArgs A) {
@@ -207,8 +209,10 @@ hipError_t launch_batch_init(const QrBatchArgs &a, hipStream_t s) {
 hipError_t launch_batch_pass(bool even, const QrBatchArgs &a, hipStream_t s) {
     if (a.n_tiles <= 0) return hipSuccess;
     const int grid = batch_pass_grid(a.n_tiles);
-    auto k = a.zf ? (even ? batch_pass_kernel<true, true> : batch_pass_kernel<false, true>)
-                  : (even ? batch_pass_kernel<true, false> : batch_pass_kernel<false, false>);
+    auto k = a.zf ? (even ? batch_pass_kernel<true, true, kBatchVec, kBatchLpc>
+                          : batch_pass_kernel<false, true, kBatchVec, kBatchLpc>)
+                  : (even ? batch_pass_kernel<true, false, kBatchVec, kBatchLpc>
+                          : batch_pass_kernel<false, false, kBatchVec, kBatchLpc>);
     hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }

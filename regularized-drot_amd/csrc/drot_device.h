@@ -117,9 +117,20 @@ __device__ __forceinline__ double xor16_add(double v) {
 // change the bits).  All in registers: the row swaps for offsets 32 and 16,
 // row_ror:8 for xor 8 within a 16-lane row, row_ror:4 for xor 4 once the
 // values repeat every 8 lanes (after the xor-8 step), quad_perm for 2 and 1.
+// An 8-lane segment (WIDTH == 8) starts with row_half_mirror, which pairs lane
+// i with lane 7 - i of its half-row: afterwards the lanes i and 7 - i hold the
+// same sum (an addition commutes bit for bit), so the quad steps that follow
+// leave the same total in all eight lanes.
 template <int WIDTH>
 __device__ __forceinline__ double seg_sum(double v) {
-    static_assert(WIDTH == 16 || WIDTH == 32 || WIDTH == 64, "segment of 16, 32 or 64 lanes");
+    static_assert(WIDTH == 8 || WIDTH == 16 || WIDTH == 32 || WIDTH == 64,
+                  "segment of 8, 16, 32 or 64 lanes");
+    if constexpr (WIDTH == 8) {
+        v += dpp_move<0x141>(v);  // row_half_mirror
+        v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+        v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+        return v;
+    }
     if constexpr (WIDTH >= 64) v = xor32_add(v);
     if constexpr (WIDTH >= 32) v = xor16_add(v);
     v += dpp_move<0x128>(v);  // row_ror:8
@@ -189,7 +200,7 @@ struct IterStart {
     float gamma;        // gamma of iteration k-1: phi = (hat + gamma) / n
 };
 
-// Prologue of the pass of iteration k, run by every thread of every block;
+// Prologue of the pass of iteration k, called by every thread of every block;
 // s_red holds 3 * kWavesPerBlock doubles.
 // Finishes iteration k-1 from the per-block partials of its finalize launch,
 // identically in every block (same fixed-order sums):
@@ -210,59 +221,67 @@ __device__ inline IterStart iteration_prologue(const IterCtl &C, double *s_red,
     bool conv = false;
     bool finish = k >= C.max_iters;
     if (k > 0) {
-        const float alpha_prev = st->alpha_slot[k & 1];  // after iteration k-2 (loaded early)
-        // every thread loads at most one finalize block's scalars; per-wave
-        // sums meet in LDS behind one barrier and every thread adds the wave
-        // values in the same fixed order (the scratch is next written after
-        // later barriers of the pass)
-        double rr = 0.0, ss = 0.0, oo = 0.0;
-        for (int t = threadIdx.x; t < C.n_fin; t += kBlock) {
-            rr += C.pfin[3 * t];
-            ss += C.pfin[3 * t + 1];
-            oo += C.pfin[3 * t + 2];
-        }
-        rr = wave_sum(rr);
-        ss = wave_sum(ss);
-        oo = wave_sum(oo);
-        const int wv = threadIdx.x / kWave;
-        if ((threadIdx.x & (kWave - 1)) == 0) {
-            s_red[wv] = rr;
-            s_red[kWavesPerBlock + wv] = ss;
-            s_red[2 * kWavesPerBlock + wv] = oo;
+        // Wave 0 alone reduces the finalize partials and forms the scalars;
+        // the block's other waves take gamma and the stop decision from two
+        // LDS slots behind one barrier.  Lane l adds the entries l, l + 64,
+        // l + 128, ... in that order (their loads in flight together), then a
+        // butterfly: the same fixed order in every block.  Slots 2W and 2W+1
+        // (W = kWavesPerBlock) are ones that block_sum, the only later user of
+        // s_red in a pass, never writes.
+        if (threadIdx.x < kWave) {
+            const float alpha_prev = st->alpha_slot[k & 1];  // after iteration k-2
+            double rr = 0.0, ss = 0.0, oo = 0.0;
+            for (int t0 = (int)threadIdx.x; t0 < C.n_fin; t0 += 4 * kWave) {
+                double v[4][3];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int t = t0 + u * kWave;
+                    const bool ok = t < C.n_fin;
+                    v[u][0] = ok ? C.pfin[3 * t] : 0.0;
+                    v[u][1] = ok ? C.pfin[3 * t + 1] : 0.0;
+                    v[u][2] = ok ? C.pfin[3 * t + 2] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    rr += v[u][0];
+                    ss += v[u][1];
+                    oo += v[u][2];
+                }
+            }
+            rr = wave_sum(rr);
+            ss = wave_sum(ss);
+            oo = wave_sum(oo);
+            const double S = ss;
+            const double obj = oo;
+            const double res = sqrt(rr);
+            const float gamma =
+                (float)((2.0 * (S - 1.0) - (double)alpha_prev) / (double)(C.n_rows + C.n_cols));
+            const float alpha_new = (float)(((double)alpha_prev + 1.0) - S);
+            const bool prev_even = ((k - 1) & 1) == 0;
+            const bool every = C.stop_rule == kStopEveryLeq;
+            const bool check = every || prev_even;
+            const bool has_obj = !C.obj_even_only || prev_even;
+            conv = check && (every ? (res <= (double)C.eps) : (res < (double)C.eps));
+            if (leader) {
+                st->alpha_slot[(k - 1) & 1] = alpha_new;
+                st->alpha = alpha_new;
+                st->gamma = gamma;
+                st->phi_mode = 0;
+                st->xsum = S;
+                st->res_last = res;
+                if (check) st->res = res;
+                if (has_obj) st->obj = obj;
+                st->last_k = k - 1;
+            }
+            if (threadIdx.x == 0) {
+                s_red[2 * kWavesPerBlock] = (double)gamma;
+                s_red[2 * kWavesPerBlock + 1] = (finish || conv) ? 1.0 : 0.0;
+            }
         }
         __syncthreads();
-        rr = ss = oo = 0.0;
-#pragma unroll
-        for (int q = 0; q < kWavesPerBlock; q++) {
-            rr += s_red[q];
-            ss += s_red[kWavesPerBlock + q];
-            oo += s_red[2 * kWavesPerBlock + q];
-        }
-        const double S = ss;
-        const double obj = oo;
-        const double res = sqrt(rr);
-        const float gamma =
-            (float)((2.0 * (S - 1.0) - (double)alpha_prev) / (double)(C.n_rows + C.n_cols));
-        const float alpha_new = (float)(((double)alpha_prev + 1.0) - S);
-        const bool prev_even = ((k - 1) & 1) == 0;
-        const bool every = C.stop_rule == kStopEveryLeq;
-        const bool check = every || prev_even;
-        const bool has_obj = !C.obj_even_only || prev_even;
-        conv = check && (every ? (res <= (double)C.eps) : (res < (double)C.eps));
-        finish = finish || conv;
-        if (leader) {
-            st->alpha_slot[(k - 1) & 1] = alpha_new;
-            st->alpha = alpha_new;
-            st->gamma = gamma;
-            st->phi_mode = 0;
-            st->xsum = S;
-            st->res_last = res;
-            if (check) st->res = res;
-            if (has_obj) st->obj = obj;
-            st->last_k = k - 1;
-        }
         r.explicit_phi = false;
-        r.gamma = gamma;
+        r.gamma = (float)s_red[2 * kWavesPerBlock];
+        finish = s_red[2 * kWavesPerBlock + 1] != 0.0;
     }
     if (finish && leader) {
         st->converged = conv ? 1 : 0;

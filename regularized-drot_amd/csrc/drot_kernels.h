@@ -80,10 +80,10 @@ struct InitArgs {
 };
 
 struct QrTile {
-    int vec;     // 1 or 4 floats per lane access
-    int lpc;     // lanes per column: 16, 32 or 64
+    int vec;     // rows of a column per lane: 1 or 4 (one access), 8 (two 4-row chunks)
+    int lpc;     // lanes per column: 16, 32 or 64 (vec 1, 4); 8 or 16 (vec 8)
     int nt = 0;  // X store policy: 0 default (L2 write-back), 2 write-through (sc1)
-    int u = 4;   // column steps per load batch (2 or 4)
+    int u = 4;   // column steps per load batch (2 or 4; 1 for vec 8)
 };
 
 struct GlTile {
@@ -165,10 +165,14 @@ struct QrBatchArgs {
     long long zf_len;
 };
 
-// Batched launches use LPC = 16 lanes per column, 16-byte accesses (tile rows
-// 64) and LPC column steps per wave (256 columns per tile).
+// Batched tiles are 64 rows by 256 columns: 16 lanes per column holding 4
+// rows each, 4 columns per wave step, 16 column steps.  (Two 4-row chunks per
+// lane on 8 lanes, the same tile, made the mixed 256-problem forward 4%
+// slower: 0.874 vs 0.837 s; its occupancy is 4 waves per SIMD against 5.)
+constexpr int kBatchVec = 4;
 constexpr int kBatchLpc = 16;
-constexpr int kBatchTileRows = kBatchLpc * 4;
+constexpr int kBatchTileRows = kBatchLpc * kBatchVec;
+constexpr int kBatchTileCols = 256;
 
 int batch_pass_grid(int n_tiles);
 hipError_t launch_batch_init(const QrBatchArgs &a, hipStream_t s);

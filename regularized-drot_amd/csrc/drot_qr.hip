@@ -62,9 +62,15 @@ __global__ __launch_bounds__(kBlock) void qr_pass_kernel(QrPassArgs A) {
 
 // ----------------------------------------------------------------- launch
 
+// vec 1 and 4: one chunk per lane, 16 to 64 lanes per column, 2 or 4 column
+// steps per load batch; vec 8: two 4-row chunks per lane, 8 or 16 lanes per
+// column, one step per batch (a batch then holds as many bytes per lane as
+// two steps of vec 4), for row counts that are a multiple of 4.
 bool qr_tile_supported(const QrTile &t) {
+    if (t.nt != 0 && t.nt != 2) return false;
+    if (t.vec == 8) return (t.lpc == 8 || t.lpc == 16) && t.u == 1;
     return (t.vec == 1 || t.vec == 4) && (t.lpc == 16 || t.lpc == 32 || t.lpc == 64) &&
-           (t.nt == 0 || t.nt == 2) && (t.u == 2 || t.u == 4);
+           (t.u == 2 || t.u == 4);
 }
 
 // Blocks of kernel k the current device holds at once (0 when the query
@@ -132,6 +138,13 @@ static void launch_pass_v(const QrTile &t, bool even, int grid, const QrPassArgs
     }
 }
 
+template <int LPC>
+static void launch_pass_v8(const QrTile &t, bool even, int grid, const QrPassArgs &a,
+                           hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    if (t.nt == 2) launch_pass_t<8, LPC, 2, 1>(even, grid, a, s, e0, e1);
+    else launch_pass_t<8, LPC, 0, 1>(even, grid, a, s, e0, e1);
+}
+
 int qr_pass_grid(int n_rt, int n_ct) {
     const int per = (n_rt * n_ct + kXcds - 1) / kXcds;
     return per * kXcds;
@@ -139,7 +152,10 @@ int qr_pass_grid(int n_rt, int n_ct) {
 
 hipError_t launch_qr_pass(const QrTile &t, bool even, int grid, const QrPassArgs &a,
                           hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    if (t.vec == 4) {
+    if (t.vec == 8) {
+        if (t.lpc == 8) launch_pass_v8<8>(t, even, grid, a, s, e0, e1);
+        else launch_pass_v8<16>(t, even, grid, a, s, e0, e1);
+    } else if (t.vec == 4) {
         if (t.lpc == 16) launch_pass_v<4, 16>(t, even, grid, a, s, e0, e1);
         else if (t.lpc == 32) launch_pass_v<4, 32>(t, even, grid, a, s, e0, e1);
         else launch_pass_v<4, 64>(t, even, grid, a, s, e0, e1);

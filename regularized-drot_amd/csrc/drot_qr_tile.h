@@ -151,6 +151,23 @@ __device__ __forceinline__ unsigned long long group_bits(unsigned long long b, i
 // partial.  The objective's wave sums meet behind the same barrier (in their
 // own LDS slots: s_red may still be read by the prologue of a slower wave) and
 // are added in wave order as block_sum does.
+// Rows of a lane's slice.  A lane holds VEC rows of a column as NCH chunks of
+// CH = min(VEC, 4) consecutive rows; chunk h of lane rl starts at tile row
+// h*CH*LPC + rl*CH, so one 16-byte access per chunk covers CH*LPC contiguous
+// rows of the column across the column group (VEC <= 4: one chunk, rows
+// rl*VEC ... rl*VEC + VEC-1).
+template <int VEC>
+struct LaneSlice {
+    static constexpr int CH = VEC < 4 ? VEC : 4;
+    static constexpr int NCH = VEC / CH;
+};
+
+template <int VEC, int LPC>
+__device__ __forceinline__ int tile_row(int rl, int e) {
+    constexpr int CH = LaneSlice<VEC>::CH;
+    return (e / CH) * (CH * LPC) + rl * CH + (e % CH);
+}
+
 template <int VEC, int LPC, bool EVEN>
 __device__ __forceinline__ void tile_partials_out(const QrTileJob &J, const double (&racc)[VEC],
                                                   double oacc, double (*s_row)[LPC * VEC]) {
@@ -162,7 +179,7 @@ __device__ __forceinline__ void tile_partials_out(const QrTileJob &J, const doub
     const int slot = w * CPW + lane / LPC;
     const int rl = lane % LPC;
 #pragma unroll
-    for (int e = 0; e < VEC; e++) s_row[slot][rl * VEC + e] = racc[e];
+    for (int e = 0; e < VEC; e++) s_row[slot][tile_row<VEC, LPC>(rl, e)] = racc[e];
     if (EVEN) {
         const double ow = wave_sum(oacc);
         if (lane == 0) s_obj[w] = ow;
@@ -209,7 +226,7 @@ constexpr int kLeanColSlack = 64;
 
 template <int VEC>
 __device__ __forceinline__ bool qr_lean_ok(const QrTileJob &J) {
-    return (J.ld % VEC) == 0 &&
+    return (J.ld % LaneSlice<VEC>::CH) == 0 &&
            (unsigned long long)(J.tc + kLeanColSlack) * (unsigned long long)J.ld * 4ull <
                (1ull << 31);
 }
@@ -252,6 +269,35 @@ struct BufIo<1> {
     }
 };
 
+// A lane's VEC rows of one column: one access per chunk, at its own offset.
+template <int VEC>
+struct LaneIo {
+    static constexpr int CH = LaneSlice<VEC>::CH;
+    static constexpr int NCH = LaneSlice<VEC>::NCH;
+    static __device__ __forceinline__ void load(__amdgpu_buffer_rsrc_t r,
+                                                const unsigned (&off)[NCH], float (&v)[VEC]) {
+#pragma unroll
+        for (int h = 0; h < NCH; h++) {
+            float t[CH];
+            BufIo<CH>::load(r, off[h], t);
+#pragma unroll
+            for (int i = 0; i < CH; i++) v[h * CH + i] = t[i];
+        }
+    }
+    template <int POL>
+    static __device__ __forceinline__ void store(__amdgpu_buffer_rsrc_t r,
+                                                 const unsigned (&off)[NCH],
+                                                 const float (&v)[VEC]) {
+#pragma unroll
+        for (int h = 0; h < NCH; h++) {
+            float t[CH];
+#pragma unroll
+            for (int i = 0; i < CH; i++) t[i] = v[h * CH + i];
+            BufIo<CH>::template store<POL>(r, off[h], t);
+        }
+    }
+};
+
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_over(const void *p, unsigned bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, kBufDword3);
 }
@@ -278,8 +324,11 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
     const int w = wave_uniform(threadIdx.x / kWave);
     const int cs = lane / LPC;
     const int rl = lane % LPC;
-    const int row = J.rt * TR + rl * VEC;
-    const bool row_ok = row < N;
+    using IoV = LaneIo<VEC>;
+    constexpr int NCH = IoV::NCH;
+    int rows[VEC];  // problem row of each of the lane's elements
+#pragma unroll
+    for (int e = 0; e < VEC; e++) rows[e] = J.rt * TR + tile_row<VEC, LPC>(rl, e);
     const int tile_col0 = J.ct * J.tc;
     const int tile_cols = min(J.tc, M - tile_col0);
     const int jt0 = w * CPW + cs;  // tile column of step t: jt0 + t * kColStride
@@ -298,7 +347,12 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
     const __amdgpu_buffer_rsrc_t pr = buffer_over(J.pcol + part_base, (unsigned)tile_cols * 8u);
     const __amdgpu_buffer_rsrc_t zr =
         ZS ? buffer_over(J.zout + part_base, (unsigned)tile_cols) : pr;
-    const unsigned x_lane = row_ok ? ((unsigned)jt0 * (unsigned)ld + (unsigned)row) * 4u : kOob;
+    unsigned x_lane[NCH];  // a chunk starting past n_rows reads zeros, stores nothing
+#pragma unroll
+    for (int h = 0; h < NCH; h++) {
+        const int r0 = rows[h * IoV::CH];
+        x_lane[h] = r0 < N ? ((unsigned)jt0 * (unsigned)ld + (unsigned)r0) * 4u : kOob;
+    }
     const unsigned x_step = (unsigned)kColStride * (unsigned)ld * 4u;
     const unsigned h_lane = (unsigned)jt0 * 4u;
     const unsigned p_lane = rl == 0 ? (unsigned)jt0 * 8u : kOob;
@@ -324,19 +378,27 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int t = t0 + u;  // steps past nstep read columns past the tile: zeros
-            const unsigned off = x_lane + (unsigned)t * x_step;
+            unsigned off[NCH];
+#pragma unroll
+            for (int h = 0; h < NCH; h++) off[h] = x_lane[h] + (unsigned)t * x_step;
             hv[u] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
                 hr, (int)(h_lane + (unsigned)(t * kColStride) * 4u), 0, 0));
             if constexpr (ZS) {
                 const bool z = (zmask >> (t & (kZBits - 1))) & 1;
                 // a flagged segment: the even pass reads no X (+0), the odd
                 // pass reads C (and X as +0) to form what the even pass left
-                if (EVEN) BufIo<VEC>::load(cr, off, cv[u]);
-                BufIo<VEC>::load(xr, z ? kOob : off, xv[u]);
-                if (!EVEN) BufIo<VEC>::load(cr, z ? off : kOob, cv[u]);
+                unsigned xoff[NCH], coff[NCH];
+#pragma unroll
+                for (int h = 0; h < NCH; h++) {
+                    xoff[h] = z ? kOob : off[h];
+                    coff[h] = z ? off[h] : kOob;
+                }
+                if (EVEN) IoV::load(cr, off, cv[u]);
+                IoV::load(xr, xoff, xv[u]);
+                if (!EVEN) IoV::load(cr, coff, cv[u]);
             } else {
-                BufIo<VEC>::load(xr, off, xv[u]);
-                if (EVEN) BufIo<VEC>::load(cr, off, cv[u]);
+                IoV::load(xr, off, xv[u]);
+                if (EVEN) IoV::load(cr, off, cv[u]);
             }
         }
     };
@@ -344,7 +406,7 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
     float h1[VEC];
 #pragma unroll
     for (int e = 0; e < VEC; e++)
-        h1[e] = row + e < N ? (explicit_phi ? J.phi1[row + e] : J.ahat[row + e]) : 0.0f;
+        h1[e] = rows[e] < N ? (explicit_phi ? J.phi1[rows[e]] : J.ahat[rows[e]]) : 0.0f;
 
     const IterStart is = iteration_prologue(ctl, s_red, leader_block);
     if (is.finish) return;
@@ -366,7 +428,7 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
     float f1[VEC];
 #pragma unroll
     for (int e = 0; e < VEC; e++)
-        f1[e] = row + e < N ? (explicit_phi ? h1[e] : (h1[e] + gamma) / fM) : -INFINITY;
+        f1[e] = rows[e] < N ? (explicit_phi ? h1[e] : (h1[e] + gamma) / fM) : -INFINITY;
     double racc[VEC];
 #pragma unroll
     for (int e = 0; e < VEC; e++) racc[e] = 0.0;
@@ -392,35 +454,41 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
             if (EVEN && pad) {
 #pragma unroll
                 for (int e = 0; e < VEC; e++)
-                    if (row + e >= N) cv[u][e] = 0.0f;
+                    if (rows[e] >= N) cv[u][e] = 0.0f;
             }
-            float yv[VEC], xq[VEC];
+            // max(X + phi1 + phi2 - sC, 0): odd pass, flagged segment: X = +0
+            // and C read, giving fmaf(-s, c, +0); elsewhere C reads 0 and
+            // fmaf(-s, 0, x) == x
+            auto y_of = [&](int e) __attribute__((always_inline)) {
+                const float xe =
+                    (!EVEN && ZS) ? __builtin_fmaf(-s, cv[u][e], xv[u][e]) : xv[u][e];
+                const float tt = (xe + f1[e]) + f2;
+                return fmaxf(EVEN ? __builtin_fmaf(-s, cv[u][e], tt) : tt, 0.0f);
+            };
+            float xq[VEC];
             unsigned lo = 0xFFFFFFFFu, hi = 0u;
 #pragma unroll
             for (int e = 0; e < VEC; e++) {
-                // odd pass, flagged segment: X = +0, C read, giving fmaf(-s, c, +0);
-                // elsewhere C reads 0 and fmaf(-s, 0, x) == x
-                const float xe = (!EVEN && ZS) ? __builtin_fmaf(-s, cv[u][e], xv[u][e]) : xv[u][e];
-                const float tt = (xe + f1[e]) + f2;
-                const float y = EVEN ? __builtin_fmaf(-s, cv[u][e], tt) : tt;
-                yv[e] = fmaxf(y, 0.0f);
-                xq[e] = div_kappa(yv[e], kappa, rkappa);
-                const unsigned b = __float_as_uint(yv[e]);
+                const float yv = y_of(e);
+                xq[e] = div_kappa(yv, kappa, rkappa);
+                const unsigned b = __float_as_uint(yv);
                 lo = min(lo, b - 1u);  // +0 maps to the largest value
                 hi = max(hi, b);
             }
             if (__builtin_amdgcn_ballot_w64(lo < ylo_m1 || hi > yhi)) {
+                // y again from the operands (still live), so that it need not
+                // stay in registers for this rare branch
 #pragma unroll
-                for (int e = 0; e < VEC; e++)
-                    if (div_kappa_slow(yv[e], xq[e])) xq[e] = yv[e] / kappa;
+                for (int e = 0; e < VEC; e++) {
+                    const float yv = y_of(e);
+                    if (div_kappa_slow(yv, xq[e])) xq[e] = yv / kappa;
+                }
             }
-            float xo[VEC];
             double csum = 0.0;
             unsigned nzb = 0u;
 #pragma unroll
             for (int e = 0; e < VEC; e++) {
                 const float xp = xq[e];  // +0 or positive: never -0
-                xo[e] = EVEN ? __builtin_fmaf(-s, cv[u][e], xp) : xp;
                 const double d = (double)xp;
                 racc[e] += d;
                 csum = e == 0 ? d : csum + d;
@@ -429,17 +497,22 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
                 if (EVEN) oacc = __builtin_fma(d, (double)cv[u][e], oacc);
                 nzb |= __float_as_uint(xp);
             }
-            const unsigned off = x_lane + (unsigned)t * x_step;
-            unsigned st_off = off;
+            unsigned st_off[NCH];
+#pragma unroll
+            for (int h = 0; h < NCH; h++) st_off[h] = x_lane[h] + (unsigned)t * x_step;
             if constexpr (ZS) {
                 const bool zseg =
                     group_bits<LPC>(__builtin_amdgcn_ballot_w64(nzb != 0u), cs) == 0ull;
                 __builtin_amdgcn_raw_buffer_store_b8((unsigned char)(zseg ? 1 : 0), zr,
                                                      (int)(z_lane + (unsigned)(t * kColStride)),
                                                      0, 0);
-                st_off = zseg ? kOob : off;
+#pragma unroll
+                for (int h = 0; h < NCH; h++) st_off[h] = zseg ? kOob : st_off[h];
             }
-            BufIo<VEC>::template store<kPol>(xr, st_off, xo);
+            float xo[VEC];
+#pragma unroll
+            for (int e = 0; e < VEC; e++) xo[e] = EVEN ? __builtin_fmaf(-s, cv[u][e], xq[e]) : xq[e];
+            IoV::template store<kPol>(xr, st_off, xo);
             csum = seg_sum<LPC>(csum);
             const unsigned long long cb64 = (unsigned long long)__double_as_longlong(csum);
             typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
@@ -676,11 +749,18 @@ __device__ __forceinline__ void qr_tile_generic(const QrTileJob &J, const IterCt
 template <int VEC, int LPC, bool EVEN, int ST, int U, bool ZS>
 __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, bool leader_block,
                                         double (*s_row)[LPC * VEC], double *s_red) {
-    if (qr_lean_ok<VEC>(J)) {  // wave-uniform
-        qr_tile_lean<VEC, LPC, EVEN, ST, U, ZS>(J, ctl, leader_block, s_row, s_red);
-        return;
+    if constexpr (VEC == 8) {
+        // two-chunk slices exist in the lean form only; the host gives such
+        // tiles to shapes that satisfy qr_lean_ok (rows a multiple of 4)
+        if (qr_lean_ok<VEC>(J))
+            qr_tile_lean<VEC, LPC, EVEN, ST, U, ZS>(J, ctl, leader_block, s_row, s_red);
+    } else {
+        if (qr_lean_ok<VEC>(J)) {  // wave-uniform
+            qr_tile_lean<VEC, LPC, EVEN, ST, U, ZS>(J, ctl, leader_block, s_row, s_red);
+            return;
+        }
+        qr_tile_generic<VEC, LPC, EVEN, ST, U, ZS>(J, ctl, leader_block, s_row, s_red);
     }
-    qr_tile_generic<VEC, LPC, EVEN, ST, U, ZS>(J, ctl, leader_block, s_row, s_red);
 }
 
 }  // namespace drot

@@ -75,8 +75,20 @@ struct Plan {
     size_t off_zf = 0, zf_bytes = 0;
 };
 
+// Whether a supported tile may run a shape.  16-byte tiles need 16-byte
+// aligned buffers (v4); two-chunk (vec 8) tiles exist only in the branch-free
+// form, which needs every 4-row chunk inside one column (n_rows % 4 == 0) and
+// a column tile, with the load batches' slack, under 2^31 bytes.
+bool qr_tile_fits(const QrTile &t, int n_rows, int nstep, bool v4) {
+    if (t.vec == 1) return true;
+    if (!v4) return false;
+    if (t.vec == 4) return true;
+    const long long tc = (long long)nstep * kWavesPerBlock * (kWave / t.lpc);
+    return n_rows % 4 == 0 && (tc + 64) * (long long)n_rows * 4 < (1LL << 31);
+}
+
 // Optional override of the QR tile, "vec,lpc,nstep[,nt[,u]]" (tuning sweeps only).
-bool qr_tile_from_env(int vec_allowed, QrTile *t, int *nstep) {
+bool qr_tile_from_env(int vec_allowed, int n_rows, QrTile *t, int *nstep) {
     const char *s = getenv("DROT_QR_TILE");
     if (!s || !*s) return false;
     int v = 0, l = 0, n = 0, nt = 0, u = 4;
@@ -84,7 +96,7 @@ bool qr_tile_from_env(int vec_allowed, QrTile *t, int *nstep) {
     if (got < 3) return false;
     QrTile cand{v, l, got >= 4 ? nt : 0, got >= 5 ? u : 4};
     if (!qr_tile_supported(cand) || n < 1) return false;
-    if (v == 4 && !vec_allowed) return false;
+    if (!qr_tile_fits(cand, n_rows, n, vec_allowed != 0)) return false;
     *t = cand;
     *nstep = n;
     return true;
@@ -217,7 +229,8 @@ bool tuned_tile(int n_rows, int n_cols, bool v4, QrTile *t, int *nstep) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tuned.find({n_rows, n_cols});
     if (it == g_tuned.end()) return false;
-    if (it->second.t.vec == 4 && !v4) return false;  // unaligned buffers: heuristic scalar tile
+    // e.g. unaligned buffers: the heuristic scalar tile
+    if (!qr_tile_fits(it->second.t, n_rows, it->second.nstep, v4)) return false;
     *t = it->second.t;
     *nstep = it->second.nstep;
     return true;
@@ -260,7 +273,8 @@ Plan plan_qr(int n_rows, int n_cols, bool vec4_ok) {
     const bool v4 = vec4_ok;
     QrTile t;
     int nstep = 0;
-    if (!qr_tile_from_env(v4, &t, &nstep) && !tuned_tile(n_rows, n_cols, v4, &t, &nstep)) {
+    if (!qr_tile_from_env(v4, n_rows, &t, &nstep) &&
+        !tuned_tile(n_rows, n_cols, v4, &t, &nstep)) {
         if (v4) {
             builtin_tile16(n_rows, n_cols, &t, &nstep);
         } else {
@@ -274,6 +288,15 @@ Plan plan_qr(int n_rows, int n_cols, bool vec4_ok) {
 // Candidate tiles of the autotuner for one shape.
 std::vector<std::pair<QrTile, int>> tune_candidates(int n_rows, int n_cols, bool v4) {
     std::vector<std::pair<QrTile, int>> out;
+    // two-chunk tiles: 8 or 16 lanes per column, one step per batch
+    for (int lpc : {8, 16})
+        for (int nt : {2, 0})
+            for (int nstep : {1, 2, 3, 4, 5, 6, 7, 8, 12, 16}) {
+                const QrTile t{8, lpc, nt, 1};
+                const int tc = nstep * kWavesPerBlock * (kWave / lpc);
+                if (lpc * 8 > 4 * std::max(n_rows, 16) || tc > 4 * std::max(n_cols, 4)) continue;
+                if (qr_tile_fits(t, n_rows, nstep, v4)) out.push_back({t, nstep});
+            }
     for (int vec : {4, 1}) {
         if (vec == 4 && !v4) continue;
         for (int lpc : {16, 32, 64}) {
@@ -1250,14 +1273,15 @@ drot_status plan_batch(int n, const drot_batch_problem *in, BatchPlan *out) {
         P.n_cols = d.n_cols;
         P.ld = d.ld;
         P.n_rt = ceil_div(d.n_rows, kBatchTileRows);
-        // 16 column steps per wave (256-column tiles): with zero-segment flags
-        // a block's cost is mostly its latency chain (flag load, first data,
-        // prologue), which longer tiles spread over more columns.  Mixed
-        // 256-problem forward: 1.21 s at 4 steps, 1.04 s at 8, 0.99 s at 16
-        // (16 is the most a 16-lane column group gathers flags for).
-        P.nstep = kBatchLpc;
-        const int tc = P.nstep * tc_min;
-        if ((long long)tc * d.ld * 4 >= (1LL << 31))
+        // 256-column tiles: with zero-segment flags a block's cost is mostly
+        // its latency chain (flag load, first data, prologue), which longer
+        // tiles spread over more columns (mixed 256-problem forward with 4-row
+        // lane slices: 1.21 s at 64 columns, 1.04 s at 128, 0.99 s at 256).
+        // The tile's column range plus the load batches' slack of 64 columns
+        // must stay under 2^31 bytes (the branch-free tile's buffer offsets).
+        P.nstep = kBatchTileCols / tc_min;
+        const int tc = kBatchTileCols;
+        if ((long long)(tc + 64) * d.ld * 4 >= (1LL << 31))
             return fail(DROT_ERR_UNSUPPORTED, "problem " + std::to_string(i) + ": ld too large");
         P.n_ct = ceil_div(d.n_cols, tc);
         P.prow_off = prow_off;
@@ -1966,7 +1990,8 @@ drot_status drot_qr_tile_cache_set(int n_rows, int n_cols, const drot_qr_tile *t
     if (!tile || n_rows <= 0 || n_cols <= 0)
         return fail(DROT_ERR_INVALID_ARGUMENT, "bad arguments to drot_qr_tile_cache_set");
     const QrTile t{tile->vec, tile->lanes_per_col, tile->store_policy, tile->unroll};
-    if (!qr_tile_supported(t) || tile->col_steps < 1 || tile->col_steps > 64)
+    if (!qr_tile_supported(t) || tile->col_steps < 1 || tile->col_steps > 64 ||
+        !qr_tile_fits(t, n_rows, tile->col_steps, true))
         return fail(DROT_ERR_INVALID_ARGUMENT, "unsupported tile");
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned[{n_rows, n_cols}] = TunedTile{t, tile->col_steps, 0.0};
@@ -2009,7 +2034,8 @@ drot_status drot_qr_tile_cache_load(const char *path) {
         const QrTile t{v, l, nt, u};
         // 16-byte tiles serve every row count (a column's partial last chunk
         // is accessed element by element), as drot_qr_tile_cache_set accepts
-        if (nr <= 0 || nc <= 0 || !qr_tile_supported(t) || ns < 1 || ns > 64)
+        if (nr <= 0 || nc <= 0 || !qr_tile_supported(t) || ns < 1 || ns > 64 ||
+            !qr_tile_fits(t, nr, ns, true))
             return fail(DROT_ERR_INVALID_ARGUMENT, std::string(path) + ":" + std::to_string(lineno) +
                                                        ": unsupported tile");
         got[{nr, nc}] = TunedTile{t, ns, us};

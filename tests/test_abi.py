@@ -177,6 +177,25 @@ def test_tile_cache_rejects_bad_tiles(tmp_path):
         _abi.qr_tile_cache_load(str(bad))
 
 
+def test_two_chunk_tiles_need_rows_multiple_of_4(monkeypatch):
+    """vec 8 tiles (two 4-row chunks per lane) run only the branch-free pass:
+    rows a multiple of 4, 16-byte aligned buffers, 8 or 16 lanes per column,
+    one column step per batch."""
+    from reg_drot import _abi
+    _abi.qr_tile_cache_clear()
+    _abi.qr_tile_cache_set(1004, 641, _abi.QrTile(8, 8, 8, 2, 1))
+    assert _abi.qr_tile_for(1004, 641).as_tuple() == (8, 8, 8, 2, 1)
+    assert _abi.qr_tile_for(1004, 641, vec4_ok=False).vec == 1  # unaligned buffers
+    for bad_rows, tile in ((1003, (8, 8, 8, 2, 1)), (1004, (8, 32, 8, 2, 1)),
+                           (1004, (8, 8, 8, 2, 2))):
+        with pytest.raises(_abi.DrotError, match="unsupported"):
+            _abi.qr_tile_cache_set(bad_rows, 641, _abi.QrTile(*tile))
+    _abi.qr_tile_cache_clear()
+    monkeypatch.setenv("DROT_QR_TILE", "8,16,5,0,1")
+    assert _abi.qr_tile_for(1004, 641).as_tuple() == (8, 16, 5, 0, 1)
+    assert _abi.qr_tile_for(1003, 641).vec != 8
+
+
 def test_batch_layout_is_dense_and_line_aligned():
     """drot_batch_layout: columns start on 128-byte lines (ld a multiple of 32
     floats), problems follow each other without gaps, p and q are packed."""

@@ -58,6 +58,19 @@ def tuned_tiles(tmp_path):
     tune.load(str(saved))
 
 
+def _table_tile(n_rows, n_cols):
+    """The shape's entry of the shipped tile table (vec, lanes_per_col,
+    col_steps, store_policy, unroll)."""
+    from reg_drot import tune
+    for ln in open(tune.DEFAULT_TABLE):
+        if ln.startswith("#") or not ln.strip():
+            continue
+        f = [int(v) for v in ln.split()[:7]]
+        if (f[0], f[1]) == (n_rows, n_cols):
+            return tuple(f[2:7])
+    raise AssertionError(f"no {n_rows}x{n_cols} entry in {tune.DEFAULT_TABLE}")
+
+
 # ----------------------------------------------------------------- QR 3000^2
 
 @pytest.mark.timeout(600)
@@ -66,7 +79,7 @@ def test_qr_3000_full_solve_with_tuned_tile(device, tuned_tiles, r_weight):
     from reg_drot import _abi, tune
     from test_gpu_qr import gpu_qr
     n = 3000
-    assert tune.tile_for(n, n) == (4, 16, 7, 2, 2)
+    assert tune.tile_for(n, n) == _table_tile(n, n)
     c = problems.cloud_cost(n, n, dim=16, seed=0)
     p, q = problems.uniform_marginals(n, n)
     prm = _abi.qr_params_from_rho(2.0, r_weight, n, n, 20000, 1e-4)
@@ -305,7 +318,7 @@ def test_qr_10000_trajectory_with_tuned_tile(device, tuned_tiles, qr10000):
     from reg_drot import _abi, tune
     from test_gpu_qr import gpu_qr
     n = 10000
-    assert tune.tile_for(n, n) == (4, 64, 16, 0, 2)
+    assert tune.tile_for(n, n) == _table_tile(n, n)
     c, p, q = qr10000
     s32, k32 = O.qr_params_torch(2.0, 1e-3, n, n)
     st = O.warm_start(n, n, s32)

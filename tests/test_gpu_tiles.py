@@ -21,6 +21,8 @@ pytestmark = pytest.mark.gpu
 TILES = [(v, l, n, nt, u)
          for v, l, nt, u in itertools.product((4, 1), (16, 32, 64), (0, 2), (2, 4))
          for n in (1, 3, 8)]
+# two 4-row chunks per lane (8 or 16 lanes per column, one step per batch)
+TILES += [(8, l, n, nt, 1) for l, nt in itertools.product((8, 16), (0, 2)) for n in (1, 3, 8)]
 
 
 def _solve(c, p, q, prm, tile, device):
@@ -57,6 +59,11 @@ def test_tile_trajectory_matches_oracle(device, tile, n_rows):
     c = problems.reference_test_cost(n_rows, n_cols, seed=17)
     p, q = problems.uniform_marginals(n_rows, n_cols)
     prm = _abi.qr_params_from_rho(2.0, 1e-2, n_rows, n_cols, 9, 0.0)
+    if tile[0] == 8 and n_rows % 4:
+        # two-chunk slices need every 4-row chunk inside one column
+        with pytest.raises(_abi.DrotError, match="unsupported"):
+            _abi.qr_tile_cache_set(n_rows, n_cols, _abi.QrTile(*tile))
+        return
     print(f"[tile] {tile} {n_rows}x{n_cols} 9 iterations", flush=True)
     x, r = _solve(c, p, q, prm, tile, device)
     ref = O.qr_solve(c, p, q, np.float32(prm.step_size), np.float32(prm.scale), 9, 0.0)
@@ -64,7 +71,8 @@ def test_tile_trajectory_matches_oracle(device, tile, n_rows):
     assert _rel(x, ref.x) <= 1e-5
 
 
-@pytest.mark.parametrize("tile", [(4, 64, 2, 2, 2), (1, 32, 5, 0, 4), (4, 32, 8, 0, 2)],
+@pytest.mark.parametrize("tile", [(4, 64, 2, 2, 2), (1, 32, 5, 0, 4), (4, 32, 8, 0, 2),
+                                  (8, 8, 6, 2, 1), (8, 16, 3, 0, 1)],
                          ids=lambda t: "v%d_l%d_n%d_st%d_u%d" % t)
 def test_tile_full_solve_matches_oracle(device, tile):
     from reg_drot import _abi

@@ -102,6 +102,24 @@ def test_flags_keep_bits_every_tile(device, tile):
               _solve(c, p, q, prm, device, False, tile))
 
 
+@pytest.mark.parametrize("tile", ["8,8,8,2,1", "8,16,5,0,1", "8,8,3,0,1", "8,16,16,2,1"])
+def test_flags_keep_bits_two_chunk_tiles(device, tile, monkeypatch):
+    """Two 4-row chunks per lane (rows a multiple of 4; the last row tile is
+    partial, so one chunk of a lane can lie inside and the other past n_rows)."""
+    from reg_drot import _abi
+    n_rows, n_cols = 1004, 641
+    monkeypatch.setenv("DROT_QR_TILE", tile)
+    assert _abi.qr_tile_for(n_rows, n_cols).as_tuple() == tuple(int(v) for v in tile.split(","))
+    monkeypatch.delenv("DROT_QR_TILE")
+    c = problems.cloud_cost(n_rows, n_cols, dim=16, seed=6)
+    p, q = problems.uniform_marginals(n_rows, n_cols)
+    for iters, eps in ((157, 0.0), (20000, 1e-4)):
+        prm = _abi.qr_params_from_rho(2.0, 1e-3, n_rows, n_cols, iters, eps)
+        print(f"[zero segments] tile {tile} iters={iters}", flush=True)
+        _same(_solve(c, p, q, prm, device, True, tile),
+              _solve(c, p, q, prm, device, False, tile))
+
+
 @pytest.mark.parametrize("max_iters,eps", [(20000, 1e-4), (151, 0.0)])
 def test_batch_flags_keep_bits(device, max_iters, eps):
     """The batched solver flags every problem's segments (offsets into one
