@@ -89,10 +89,17 @@ struct QrTile {
 struct GlTile {
     int vec;        // 1 or 4
     int rch;        // row chunks per thread (per lane for the wave-per-column kernel)
-    int wave;       // 1: one wave per column (gl_wave_kernel), n_rows <= 2048
+    int wave;       // 1: one wave per column (gl_wave_kernel), n_rows <= 2048;
+                    // 2: threads over rows, four columns per round (gl_rows_kernel)
     int split = 0;  // 1: columns split into row chunks over blocks (any n_rows):
                     // gl_split_norm_kernel + gl_split_apply_kernel per iteration
 };
+
+// Floats per column slice of gl_rows_kernel's LDS (prox values, then the
+// group scales), rounded up to a 16-byte multiple for its vector stores.
+__host__ __device__ inline int gl_rows_slice(int n_rows, int n_groups) {
+    return (n_rows + n_groups + 3) & ~3;
+}
 
 hipError_t launch_init(const InitArgs &a, hipStream_t s);
 // e0/e1 (optional): events recorded at the pass kernel's start and end.

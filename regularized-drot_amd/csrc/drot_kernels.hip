@@ -292,6 +292,57 @@ __global__ __launch_bounds__(kBlock) void gl_pass_kernel(GlPassArgs A) {
     if (A.ctl.tspan) record_span(A.ctl.tspan, t0);
 }
 
+// Group scales of one column from its prox values y[0 .. n_rows) in LDS, by
+// one wave, written to sc[0 .. G): one lane per group summing its rows in
+// order (groups under 64 rows), or the whole wave per group, lane l adding
+// rows lo + l, lo + l + 64, ... before a butterfly.  Fixed order either way.
+// (LDS reads are issued kNormBatch at a time; the additions keep their
+// sequential order.)
+template <typename Bnd>
+__device__ __forceinline__ void column_group_scales(const float *y, float *sc, int G,
+                                                    bool lane_per_group, const Bnd &bnd,
+                                                    double lam, int lane) {
+    if (lane_per_group) {
+        for (int g = lane; g < G; g += kWave) {
+            const int hi = bnd(g + 1);
+            int i = bnd(g);
+            double n2 = 0.0;
+            for (; i + kNormBatch <= hi; i += kNormBatch) {
+                float v[kNormBatch];
+#pragma unroll
+                for (int u = 0; u < kNormBatch; u++) v[u] = y[i + u];
+#pragma unroll
+                for (int u = 0; u < kNormBatch; u++) n2 += (double)v[u] * (double)v[u];
+            }
+            for (; i < hi; i++) n2 += (double)y[i] * (double)y[i];
+            sc[g] = group_scale(n2, lam);
+        }
+    } else {
+        // lane g % 64 keeps group g's total; the scales of up to 64 groups
+        // are formed at once
+        double mine = 0.0;
+        for (int g = 0; g < G; g++) {
+            const int hi = bnd(g + 1);
+            int i = bnd(g) + lane;
+            double n2 = 0.0;
+            for (; i + (kNormBatch - 1) * kWave < hi; i += kNormBatch * kWave) {
+                float v[kNormBatch];
+#pragma unroll
+                for (int u = 0; u < kNormBatch; u++) v[u] = y[i + u * kWave];
+#pragma unroll
+                for (int u = 0; u < kNormBatch; u++) n2 += (double)v[u] * (double)v[u];
+            }
+            for (; i < hi; i += kWave) n2 += (double)y[i] * (double)y[i];
+            n2 = wave_sum(n2);
+            if ((g & (kWave - 1)) == lane) mine = n2;
+            if ((g & (kWave - 1)) == kWave - 1 || g == G - 1) {
+                const int gb = g & ~(kWave - 1);
+                if (gb + lane <= g) sc[gb + lane] = group_scale(mine, lam);
+            }
+        }
+    }
+}
+
 // GL pass with one wave per column (n_rows <= 2048): the 4 waves of a block
 // take different columns at once, so a block never waits for a column-wide
 // barrier.  A wave keeps its column's prox values in registers and a copy in
@@ -394,48 +445,7 @@ __global__ __launch_bounds__(kBlock) void gl_wave_kernel(GlPassArgs A) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // group norms -> scales (fixed order), in this wave's LDS slice
-        // (LDS reads are issued kNormBatch at a time; the additions keep their
-        // sequential order)
-        if (A.lane_per_group) {
-            for (int g = lane; g < G; g += kWave) {
-                const int hi = bnd(g + 1);
-                int i = bnd(g);
-                double n2 = 0.0;
-                for (; i + kNormBatch <= hi; i += kNormBatch) {
-                    float v[kNormBatch];
-#pragma unroll
-                    for (int u = 0; u < kNormBatch; u++) v[u] = s_y[i + u];
-#pragma unroll
-                    for (int u = 0; u < kNormBatch; u++) n2 += (double)v[u] * (double)v[u];
-                }
-                for (; i < hi; i++) n2 += (double)s_y[i] * (double)s_y[i];
-                s_sc[g] = group_scale(n2, lam);
-            }
-        } else {
-            // wave-wide sums per group; lane g % 64 keeps group g's total and
-            // the scales of up to 64 groups are formed at once
-            double mine = 0.0;
-            for (int g = 0; g < G; g++) {
-                const int hi = bnd(g + 1);
-                int i = bnd(g) + lane;
-                double n2 = 0.0;
-                for (; i + (kNormBatch - 1) * kWave < hi; i += kNormBatch * kWave) {
-                    float v[kNormBatch];
-#pragma unroll
-                    for (int u = 0; u < kNormBatch; u++) v[u] = s_y[i + u * kWave];
-#pragma unroll
-                    for (int u = 0; u < kNormBatch; u++) n2 += (double)v[u] * (double)v[u];
-                }
-                for (; i < hi; i += kWave) n2 += (double)s_y[i] * (double)s_y[i];
-                n2 = wave_sum(n2);
-                if ((g & (kWave - 1)) == lane) mine = n2;
-                if ((g & (kWave - 1)) == kWave - 1 || g == G - 1) {
-                    const int gb = g & ~(kWave - 1);
-                    if (gb + lane <= g) s_sc[gb + lane] = group_scale(mine, lam);
-                }
-            }
-        }
+        column_group_scales(s_y, s_sc, G, A.lane_per_group != 0, bnd, lam, lane);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -482,6 +492,180 @@ __global__ __launch_bounds__(kBlock) void gl_wave_kernel(GlPassArgs A) {
                 for (int q = 0; q < kWavesPerBlock; q++) sum += cb[q][t];
                 A.prow[(size_t)blockIdx.x * N + i] = sum;
             }
+        }
+    }
+    const double o = block_sum(oacc, s_red);
+    if (threadIdx.x == 0) A.pobj[blockIdx.x] = o;
+    if (A.ctl.tspan) record_span(A.ctl.tspan, t0);
+}
+
+// GL pass with the block's threads over rows and its columns taken four at a
+// time (n_rows <= kBlock*VEC*RCH, at most 2048).  Thread t owns the rows
+// (r*kBlock + t)*VEC + e of every column of the block, so its row partials
+// add up in registers over the columns in order and are stored as they are:
+// no combine across waves.  Per round of four columns:
+//   * every thread forms y = max(X + phi1 + phi2 - sC, 0) of its rows of the
+//     four columns and writes them to the columns' LDS slices (one barrier);
+//   * wave w forms the group scales of column w from its slice in the fixed
+//     order of column_group_scales (one barrier);
+//   * every thread scales and stores its elements; the column sums and the
+//     objective are wave sums added in wave order (one barrier).
+// All loads of the first round (X, C, both potentials, the groups of the
+// rows, the done flag) are issued before the bounds staging and the
+// prologue's reductions.  Same arithmetic as gl_pass_kernel.
+template <int VEC, int RCH>
+__global__ __launch_bounds__(kBlock) void gl_rows_kernel(GlPassArgs A) {
+    constexpr int kCols = kWavesPerBlock;  // columns per round, one per wave for the norms
+    // per column of a round: y[n_rows], scale[n_groups] (gl_rows_slice floats)
+    extern __shared__ __attribute__((aligned(16))) float s_cols[];
+    __shared__ double s_red[3 * kWavesPerBlock];
+    __shared__ double s_cs[kWavesPerBlock][kCols + 1];
+    __shared__ int s_bnd[kGlBoundsLds + 1];
+    __shared__ int s_done;
+    const unsigned long long t0 = A.ctl.tspan ? (unsigned long long)wall_clock64() : 0ull;
+    const int N = A.n_rows, M = A.n_cols, G = A.n_groups;
+    const int slice = gl_rows_slice(N, G);  // floats per column slice (16-byte multiple)
+    const bool explicit_phi = A.ctl.k == 0;
+    const float *hsrc = explicit_phi ? A.phi1 : A.ahat;
+    const float *hat2 = explicit_phi ? A.phi2 : A.bhat;
+    const int j0 = blockIdx.x * A.tc;
+    const int j1 = min(j0 + A.tc, M);
+    if (threadIdx.x == 0) s_done = A.ctl.st->done;
+    float h1[RCH][VEC];
+    int gid[RCH][VEC];
+#pragma unroll
+    for (int r = 0; r < RCH; r++) {
+        const int i0 = (r * kBlock + (int)threadIdx.x) * VEC;
+#pragma unroll
+        for (int e = 0; e < VEC; e++) { h1[r][e] = 0.0f; gid[r][e] = 0; }
+        if (i0 < N) {
+            VecIO<VEC>::load(hsrc + i0, h1[r]);
+            load_ints<VEC>(A.row_group + i0, gid[r]);
+        }
+    }
+    // one register set: a second one, to keep the next round's loads in
+    // flight, took the register count from 168 to 246 and made the pass
+    // slower at every shape measured
+    float xv[kCols][RCH][VEC], cv[kCols][RCH][VEC], h2[kCols];
+    auto load_round = [&](int jr) __attribute__((always_inline)) {
+#pragma unroll
+        for (int jj = 0; jj < kCols; jj++) {
+            const int j = jr + jj;
+            const bool col_ok = j < j1;
+            h2[jj] = col_ok ? hat2[j] : 0.0f;
+            const float *xc = A.x + (size_t)j * N;
+            const float *cc = A.c + (size_t)j * N;
+#pragma unroll
+            for (int r = 0; r < RCH; r++) {
+                const int i0 = (r * kBlock + (int)threadIdx.x) * VEC;
+                if (col_ok && i0 < N) {
+                    VecIO<VEC>::load(xc + i0, xv[jj][r]);
+                    VecIO<VEC>::load(cc + i0, cv[jj][r]);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < VEC; e++) { xv[jj][r][e] = 0.0f; cv[jj][r][e] = 0.0f; }
+                }
+            }
+        }
+    };
+    load_round(j0);
+    stage_bounds(A, s_bnd);  // its barrier also publishes s_done
+    if (s_done) return;      // the solve ended before this launch
+    const IterStart is = iteration_prologue(A.ctl, s_red, blockIdx.x == 0);
+    if (is.finish) return;
+    const float gamma = is.gamma;
+    const bool bnd_lds = G <= kGlBoundsLds;
+    auto bnd = [&](int g) { return bnd_lds ? s_bnd[g] : A.bounds[g]; };
+    const float fM = (float)M, fN = (float)N;
+    const float s = A.step;
+    const double lam = (double)A.lambda;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = wave_uniform(threadIdx.x / kWave);
+
+    float f1[RCH][VEC];
+    double racc[RCH][VEC];
+#pragma unroll
+    for (int r = 0; r < RCH; r++) {
+#pragma unroll
+        for (int e = 0; e < VEC; e++) {
+            racc[r][e] = 0.0;
+            f1[r][e] = is.explicit_phi ? h1[r][e] : (h1[r][e] + gamma) / fM;
+        }
+    }
+    double oacc = 0.0;
+    for (int jr = j0; jr < j1; jr += kCols) {
+        if (jr != j0) load_round(jr);
+        // prox values of the round's columns -> their LDS slices
+#pragma unroll
+        for (int jj = 0; jj < kCols; jj++) {
+            const float f2 = is.explicit_phi ? h2[jj] : (h2[jj] + gamma) / fN;
+            float *ys = s_cols + jj * slice;
+#pragma unroll
+            for (int r = 0; r < RCH; r++) {
+                const int i0 = (r * kBlock + (int)threadIdx.x) * VEC;
+                float yv[VEC];
+#pragma unroll
+                for (int e = 0; e < VEC; e++) {
+                    const float tt = (xv[jj][r][e] + f1[r][e]) + f2;
+                    yv[e] = fmaxf(__builtin_fmaf(-s, cv[jj][r][e], tt), 0.0f);
+                    xv[jj][r][e] = yv[e];
+                }
+                if (i0 < N) VecIO<VEC>::store(ys + i0, yv);
+            }
+        }
+        __syncthreads();
+        if (jr + w < j1) {
+            float *ys = s_cols + w * slice;
+            column_group_scales(ys, ys + N, G, A.lane_per_group != 0, bnd, lam, lane);
+        }
+        __syncthreads();
+        double cs[kCols];
+#pragma unroll
+        for (int jj = 0; jj < kCols; jj++) {
+            cs[jj] = 0.0;
+            if (jr + jj >= j1) continue;  // block-uniform
+            const float *sc = s_cols + jj * slice + N;
+            float *xo_col = A.x + (size_t)(jr + jj) * N;
+#pragma unroll
+            for (int r = 0; r < RCH; r++) {
+                const int i0 = (r * kBlock + (int)threadIdx.x) * VEC;
+                if (i0 < N) {
+                    float xo[VEC];
+#pragma unroll
+                    for (int e = 0; e < VEC; e++) {
+                        const float v = xv[jj][r][e] * sc[gid[r][e]];
+                        xo[e] = v;
+                        racc[r][e] += (double)v;
+                        cs[jj] += (double)v;
+                        oacc += (double)v * (double)cv[jj][r][e];
+                    }
+                    VecIO<VEC>::store(xo_col + i0, xo);
+                }
+            }
+        }
+        // column sums: wave sums, then the waves in order
+#pragma unroll
+        for (int jj = 0; jj < kCols; jj++) cs[jj] = wave_sum(cs[jj]);
+        if (lane == 0) {
+#pragma unroll
+            for (int jj = 0; jj < kCols; jj++) s_cs[w][jj] = cs[jj];
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < kCols && jr + (int)threadIdx.x < j1) {
+            double t = 0.0;
+#pragma unroll
+            for (int q = 0; q < kWavesPerBlock; q++) t += s_cs[q][threadIdx.x];
+            A.pcol[jr + threadIdx.x] = t;
+        }
+        // the next round writes s_cs only after two more barriers, and its
+        // LDS slices only after every thread has passed this one
+    }
+#pragma unroll
+    for (int r = 0; r < RCH; r++) {
+#pragma unroll
+        for (int e = 0; e < VEC; e++) {
+            const int i = (r * kBlock + (int)threadIdx.x) * VEC + e;
+            if (i < N) A.prow[(size_t)blockIdx.x * N + i] = racc[r][e];
         }
     }
     const double o = block_sum(oacc, s_red);
@@ -846,6 +1030,21 @@ hipError_t launch_gl_pass(const GlTile &t, int grid, size_t lds, const GlPassArg
         hipError_t er = hipGetLastError();
         if (er == hipSuccess && e0 && e1) er = hipEventRecord(e1, s);
         return er;
+    }
+    if (t.wave == 2) {
+#define DROT_GLR_CASE(V, R)                                                               \
+        if (t.vec == V && t.rch == R) {                                                   \
+            if (e0 && e1)                                                                 \
+                hipExtLaunchKernelGGL((gl_rows_kernel<V, R>), dim3(grid), dim3(kBlock), lds, s, \
+                                      e0, e1, 0, a);                                      \
+            else                                                                          \
+                hipLaunchKernelGGL((gl_rows_kernel<V, R>), dim3(grid), dim3(kBlock), lds, s, a); \
+            return hipGetLastError();                                                     \
+        }
+        DROT_GLR_CASE(4, 1) DROT_GLR_CASE(4, 2)
+        DROT_GLR_CASE(1, 1) DROT_GLR_CASE(1, 2) DROT_GLR_CASE(1, 4) DROT_GLR_CASE(1, 8)
+#undef DROT_GLR_CASE
+        return hipErrorInvalidValue;
     }
     if (t.wave) {
 #define DROT_GLW_CASE(V, R)                                                               \

@@ -333,16 +333,46 @@ bool plan_gl(int n_rows, int n_cols, int n_groups, bool vec4_ok, Plan *out,
     // kernel, for tests that compare the two)
     const char *wave_env = getenv("DROT_GL_WAVE");
     const bool wave_ok = !(wave_env && wave_env[0] == '0');
+    // about 256 blocks of four waves: row partials cost 8 B per row per block
+    const int cpw = std::max(1, ceil_div(n_cols, kWavesPerBlock * 256));
     const int wchunk = kWave * P.gt.vec;
     int wrch = 1;
     while (wrch < 8 && wchunk * wrch < n_rows) wrch *= 2;
     const size_t wave_lds = (size_t)kWavesPerBlock * ((size_t)n_rows + n_groups) * 4;
-    if (wave_ok && wchunk * wrch >= n_rows && wave_lds <= 48 * 1024) {
+    const bool wave_fits = wave_ok && wchunk * wrch >= n_rows && wave_lds <= 48 * 1024;
+    // columns of up to 2048 rows: threads over rows, four columns per round.
+    // With more than two rounds per block the wave-per-column kernel, whose
+    // waves drift apart instead of meeting at three barriers per round, is
+    // faster where it applies (MI355X, GL iteration: 1500x1000 15.4 -> 14.1
+    // us, 1501x999 17.4 -> 14.6, 2048x2048 23.4 -> 21.8, 2048x4000 equal,
+    // 1024x8000 37.1 -> 39.6).  DROT_GL_ROWS=0 turns it off, for tests that
+    // compare the kernels.
+    const char *rows_env = getenv("DROT_GL_ROWS");
+    const bool rows_ok = wave_ok && !(rows_env && rows_env[0] == '0') && (cpw <= 2 || !wave_fits);
+    const int rchunk = kBlock * P.gt.vec;
+    int rrch = 1;
+    while (rrch < (v4 ? 2 : 8) && rchunk * rrch < n_rows) rrch *= 2;
+    const size_t rows_lds = (size_t)kWavesPerBlock * gl_rows_slice(n_rows, n_groups) * 4;
+    if (rows_ok && rchunk * rrch >= n_rows && rows_lds <= 48 * 1024) {
+        P.gt.wave = 2;
+        P.gt.rch = rrch;
+        P.lds = rows_lds;
+        P.tc = kWavesPerBlock * cpw;
+        P.n_ct = ceil_div(n_cols, P.tc);
+        P.n_rt = 1;
+        P.grid_pass = P.n_ct;
+        P.n_obj = P.n_ct;
+        P.lane_per_group = (n_rows / std::max(1, n_groups)) < 64 ? 1 : 0;
+        P.rb = ceil_div(n_rows, kFinRows);
+        P.cb = ceil_div(n_cols, kFinRows);
+        layout(P);
+        *out = P;
+        return true;
+    }
+    if (wave_fits) {
         P.gt.wave = 1;
         P.gt.rch = wrch;
         P.lds = wave_lds;
-        // about 256 blocks: row partials cost 8 B per row per block
-        const int cpw = std::max(1, ceil_div(n_cols, kWavesPerBlock * 256));
         P.tc = kWavesPerBlock * cpw;
         P.n_ct = ceil_div(n_cols, P.tc);
         P.n_rt = 1;

@@ -1,9 +1,13 @@
 """GPU parity of GL-DROT against the CPU oracle, through the C ABI (libdrot.so).
 
 Bar: max|X_gpu - X_oracle| <= 1e-5 * max|X_oracle| after the same number of
-iterations, identical n_iter for full solves.  Covers the wave-per-column and
-block-per-column kernels and the long-column path (any n_rows: row chunks over
-blocks, group norms completed between two launches).
+iterations, identical n_iter for full solves.  Covers the four pass kernels:
+threads over rows with four columns per round (columns of up to 2048 rows and
+up to two rounds per block, or when a column is too long for one wave), one
+wave per column (wider problems; DROT_GL_ROWS=0 forces it), one block per
+column (columns beyond 2048 rows; DROT_GL_WAVE=0 forces it) and the
+long-column path (any n_rows: row chunks over blocks, group norms completed
+between two launches).
 """
 import numpy as np
 import pytest
@@ -198,3 +202,49 @@ def test_gl_split_path_full_solve_stops_like_the_oracle(device, monkeypatch):
         assert r.converged == int(ref.converged)
         assert r.n_iter == ref.n_iter
         assert rel_err(x, ref.x) <= PLAN_RTOL
+
+
+# (n_rows, n_cols, n_groups): the same layouts on the wave-per-column kernel
+WAVE_CASES = [(10, 20, 2), (257, 64, 3), (1500, 1000, 2), (600, 200, 600)]
+
+
+@pytest.mark.parametrize("n_rows,n_cols,n_groups", WAVE_CASES)
+@pytest.mark.parametrize("iters", [5, 30])
+def test_gl_wave_kernel_trajectory(device, monkeypatch, n_rows, n_cols, n_groups, iters):
+    monkeypatch.setenv("DROT_GL_ROWS", "0")
+    print(f"[gl wave traj] {n_rows}x{n_cols} G={n_groups} iters={iters}", flush=True)
+    c = problems.reference_test_cost(n_rows, n_cols, seed=n_rows + 3 * n_cols)
+    p, q = problems.uniform_marginals(n_rows, n_cols)
+    from reg_drot import _abi as abi
+    prm = abi.glr_params_from_rho(2.0, 1e-2, n_rows, n_cols, n_groups, iters, 0.0)
+    x, r, phi1, phi2, _ = gpu_gl(c, p, q, prm, device)
+    ref = O.gl_solve(c, p, q, n_groups, np.float32(prm.step_size), np.float32(prm.lambda_),
+                     iters, np.float32(0.0))
+    assert r.iterations_run == iters
+    assert rel_err(x, ref.x) <= PLAN_RTOL
+    assert rel_err(phi1, ref.state.phi1) <= PLAN_RTOL
+    assert rel_err(phi2, ref.state.phi2) <= PLAN_RTOL
+
+
+# shapes whose blocks take several rounds of columns: threads over rows with
+# two rounds (1500 x 1500) and with three (1501 rows, one float per lane, too
+# long a column for one wave), one wave per column with three columns per
+# wave (512 x 3000)
+ROUND_CASES = [(1500, 1500, 3), (1501, 2500, 2), (512, 3000, 4)]
+
+
+@pytest.mark.parametrize("n_rows,n_cols,n_groups", ROUND_CASES)
+def test_gl_several_column_rounds(device, n_rows, n_cols, n_groups):
+    iters = 30
+    print(f"[gl rounds] {n_rows}x{n_cols} G={n_groups} iters={iters}", flush=True)
+    c = problems.reference_test_cost(n_rows, n_cols, seed=n_rows + 7 * n_cols)
+    p, q = problems.uniform_marginals(n_rows, n_cols)
+    from reg_drot import _abi as abi
+    prm = abi.glr_params_from_rho(2.0, 1e-2, n_rows, n_cols, n_groups, iters, 0.0)
+    x, r, phi1, phi2, _ = gpu_gl(c, p, q, prm, device)
+    ref = O.gl_solve(c, p, q, n_groups, np.float32(prm.step_size), np.float32(prm.lambda_),
+                     iters, np.float32(0.0))
+    assert r.iterations_run == iters
+    assert rel_err(x, ref.x) <= PLAN_RTOL
+    assert rel_err(phi1, ref.state.phi1) <= PLAN_RTOL
+    assert rel_err(phi2, ref.state.phi2) <= PLAN_RTOL
