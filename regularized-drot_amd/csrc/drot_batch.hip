@@ -139,7 +139,6 @@ __global__ __launch_bounds__(kBlock) void batch_pass_kernel(QrBatchArgs A) {
 }
 
 __global__ __launch_bounds__(kBlock) void batch_finalize_kernel(QrBatchArgs A) {
-    __shared__ double s_part[kBlock / kFinRows][kFinRows];
     __shared__ double s_red[3 * kWavesPerBlock];
     const BatchFin f = A.fins[blockIdx.x];
     const BatchProblem P = A.probs[f.prob];
@@ -164,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void batch_finalize_kernel(QrBatchArgs A) {
     F.n_fin = P.n_fin;
     F.blk = f.blk;
     F.with_obj = (A.k % 2) == 0;
-    finalize_block<8>(F, s_part, s_red);
+    finalize_rows_block(F, s_red);
 }
 
 // Per problem, after the loop; grid.y = problem.  First the segments that the

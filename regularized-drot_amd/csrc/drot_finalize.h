@@ -110,4 +110,88 @@ __device__ inline void finalize_block(const FinJob &A, double (*s_part)[kFinRows
     }
 }
 
+// A thread's sum of the partials src[t * len], t < cnt, in finalize_block's
+// grouping: part q adds t = q, q + kParts, ... in order, then the parts are
+// added in order.  CHUNK loads are in flight at once (a multiple of kParts).
+template <int CHUNK>
+__device__ __forceinline__ double partial_sum_grouped(const double *src, int len, int cnt,
+                                                      bool valid) {
+    constexpr int kParts = kBlock / kFinRows;
+    static_assert(CHUNK % kParts == 0, "whole rounds of the parts");
+    double part[kParts];
+#pragma unroll
+    for (int q = 0; q < kParts; q++) part[q] = 0.0;
+    for (int t0 = 0; t0 < cnt; t0 += CHUNK) {
+        double v[CHUNK];
+#pragma unroll
+        for (int u = 0; u < CHUNK; u++)
+            v[u] = (valid && t0 + u < cnt) ? src[(size_t)(t0 + u) * len] : 0.0;
+#pragma unroll
+        for (int u = 0; u < CHUNK; u++) part[u % kParts] += v[u];
+    }
+    double tot = 0.0;
+#pragma unroll
+    for (int q = 0; q < kParts; q++) tot += part[q];
+    return tot;
+}
+
+// The same dual update with one thread per row or column, kBlock of them per
+// block: blocks [0, rb) own kBlock rows each, the rest kBlock columns.  For
+// problems whose rows and columns have few partials (the batch: at most 8 row
+// partials per 2048 columns, 32 column partials per 2048 rows), where
+// finalize_block's eight threads per row would mostly idle and its many small
+// blocks would run in several rounds.  A thread adds its partials in
+// finalize_block's grouping (part q adds the partials t = q, q + 8, ... in
+// order; then the eight parts in order), so a row's or column's sum has the
+// same bits as there.  The per-block scalars {sum rs^2 or cs^2, sum X',
+// objective slice} cover kBlock rows or columns instead of kFinRows.
+__device__ inline void finalize_rows_block(const FinJob &A, double *s_red) {
+    constexpr int kParts = kBlock / kFinRows;
+    const int N = A.n_rows, M = A.n_cols;
+    const int b = A.blk;
+    const bool is_row = b < A.rb;
+    const int idx = (is_row ? b : b - A.rb) * kBlock + (int)threadIdx.x;
+    const int len = is_row ? N : M;
+    const int cnt = is_row ? A.n_ct : A.n_rt;
+    const bool valid = idx < len;
+    const double *src = (is_row ? A.prow : A.pcol) + (valid ? idx : 0);
+    const int done = *A.done;
+    // all partial loads of a thread in flight at once: up to kParts of them in
+    // the short form (a row of a batch problem of up to 2048 columns), up to
+    // 4 kParts in the long one (a column of up to 2048 rows)
+    const double tot = cnt <= kParts ? partial_sum_grouped<kParts>(src, len, cnt, valid)
+                                     : partial_sum_grouped<4 * kParts>(src, len, cnt, valid);
+    const float marg = valid ? (is_row ? A.p[idx] : A.q[idx]) : 0.0f;
+    const float acc = valid ? (is_row ? A.a[idx] : A.b[idx]) : 0.0f;
+    double obj = 0.0;
+    if (A.with_obj) {  // this block's slice of the pass's objective partials
+        const int per = (A.n_obj + A.n_fin - 1) / A.n_fin;
+        const int o0 = b * per;
+        const int o1 = min(o0 + per, A.n_obj);
+        for (int t = o0 + threadIdx.x; t < o1; t += kBlock) obj += A.pobj[t];
+    }
+    if (done) return;  // the solve ended: leave the dual state as it is
+    double res = 0.0, S = 0.0;
+    if (valid) {
+        const double d = tot - (double)marg;
+        const float r = (float)d;
+        const float an = acc - r;
+        if (is_row) {
+            A.a[idx] = an;
+            A.ahat[idx] = an - r;
+        } else {
+            A.b[idx] = an;
+            A.bhat[idx] = an - r;
+            S = tot;
+        }
+        res = d * d;
+    }
+    block_sum3(res, S, obj, s_red, false);  // the block ends after these stores
+    if (threadIdx.x == 0) {
+        A.pfin[0] = res;
+        A.pfin[1] = S;
+        A.pfin[2] = obj;
+    }
+}
+
 }  // namespace drot

@@ -1321,8 +1321,12 @@ drot_status plan_batch(int n, const drot_batch_problem *in, BatchPlan *out) {
         pcol_off += (long long)P.n_rt * d.n_cols;
         zf_off += (long long)P.n_rt * d.n_cols;
         P.tile_off = tile_off;
-        P.rb = ceil_div(d.n_rows, kFinRows);
-        P.n_fin = P.rb + ceil_div(d.n_cols, kFinRows);
+        // one finalize thread per row or column (finalize_rows_block): the
+        // batch's rows have at most n_cols / 256 partials; 32 rows per block
+        // with eight threads each gave some 17000 mostly idle blocks per
+        // launch for the mixed 256-problem workload, 32 us per finalize
+        P.rb = ceil_div(d.n_rows, kBlock);
+        P.n_fin = P.rb + ceil_div(d.n_cols, kBlock);
         P.fin_off = fin_off;
         P.max_iters = d.params.max_iters;
         P.step = d.params.step_size;
