@@ -72,6 +72,12 @@ __global__ __launch_bounds__(kBlock) void batch_init_kernel(QrBatchArgs A) {
             A.zf[A.zf_len + P.zf_off + i] = 0;
         }
     }
+    if (A.seg_e) {  // no headroom known yet: nothing is screened before its C is read
+        const size_t nz = (size_t)P.n_rt * (size_t)P.n_cols;
+        for (size_t i = tid; i < nz; i += nthr) A.seg_e[P.zf_off + i] = INFINITY;
+        const size_t nt = (size_t)P.n_rt * (size_t)P.n_ct;
+        for (size_t i = tid; i < nt; i += nthr) A.g1[P.tile_off + i] = 0.0;
+    }
     if (tid == 0) {
         SolverState s;
         s.done = 0;
@@ -95,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void batch_init_kernel(QrBatchArgs A) {
 // One pass over every listed tile; the list holds the tiles of the active
 // problems in row-tile-major order per problem, and blocks take contiguous
 // runs of it per XCD (see qr_pass_kernel).
-template <bool EVEN, bool ZS, int VEC, int LPC>
+template <bool EVEN, bool ZS, bool SC, int VEC, int LPC>
 __global__ __launch_bounds__(kBlock) void batch_pass_kernel(QrBatchArgs A) {
     constexpr int CPW = kWave / LPC;
     constexpr int kSteps = kBatchTileCols / (kWavesPerBlock * CPW);
@@ -131,10 +137,14 @@ __global__ __launch_bounds__(kBlock) void batch_pass_kernel(QrBatchArgs A) {
     J.kappa = P.kappa;
     J.zin = ZS ? A.zf + (size_t)((A.k + 1) & 1) * A.zf_len + P.zf_off : nullptr;
     J.zout = ZS ? A.zf + (size_t)(A.k & 1) * A.zf_len + P.zf_off : nullptr;
+    J.seg_e = SC ? A.seg_e + P.zf_off : nullptr;
+    J.g1 = SC ? A.g1 + P.tile_off + bt.rt + bt.ct * P.n_rt : nullptr;
+    J.f1_prev = SC ? A.f1pb + (size_t)((A.k + 1) & 1) * A.row_len + P.row_off : nullptr;
+    J.f1_out = SC ? A.f1pb + (size_t)(A.k & 1) * A.row_len + P.row_off : nullptr;
     const IterCtl ctl = batch_ctl(A, P, prob);
     // plain X stores: a batch's footprint lives in HBM, and write-through stores
     // made the mixed 256-problem forward 5% slower (0.99 vs 0.94 s)
-    qr_tile<VEC, LPC, EVEN, 0, (VEC == 8 ? 1 : 2), ZS>(J, ctl, bt.rt == 0 && bt.ct == 0, s_row,
+    qr_tile<VEC, LPC, EVEN, 0, (VEC == 8 ? 1 : 2), ZS, SC>(J, ctl, bt.rt == 0 && bt.ct == 0, s_row,
                                                        s_red);
 }
 
@@ -208,10 +218,12 @@ hipError_t launch_batch_init(const QrBatchArgs &a, hipStream_t s) {
 hipError_t launch_batch_pass(bool even, const QrBatchArgs &a, hipStream_t s) {
     if (a.n_tiles <= 0) return hipSuccess;
     const int grid = batch_pass_grid(a.n_tiles);
-    auto k = a.zf ? (even ? batch_pass_kernel<true, true, kBatchVec, kBatchLpc>
-                          : batch_pass_kernel<false, true, kBatchVec, kBatchLpc>)
-                  : (even ? batch_pass_kernel<true, false, kBatchVec, kBatchLpc>
-                          : batch_pass_kernel<false, false, kBatchVec, kBatchLpc>);
+    auto k = a.zf ? (a.seg_e ? (even ? batch_pass_kernel<true, true, true, kBatchVec, kBatchLpc>
+                                     : batch_pass_kernel<false, true, true, kBatchVec, kBatchLpc>)
+                             : (even ? batch_pass_kernel<true, true, false, kBatchVec, kBatchLpc>
+                                     : batch_pass_kernel<false, true, false, kBatchVec, kBatchLpc>))
+                  : (even ? batch_pass_kernel<true, false, false, kBatchVec, kBatchLpc>
+                          : batch_pass_kernel<false, false, false, kBatchVec, kBatchLpc>);
     hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }

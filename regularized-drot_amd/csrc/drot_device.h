@@ -142,6 +142,84 @@ __device__ __forceinline__ double seg_sum(double v) {
 
 __device__ __forceinline__ double wave_sum(double v) { return seg_sum<kWave>(v); }
 
+// Maximum over aligned segments of `width` lanes (8, 16, 32 or 64), in every
+// lane of the segment; same lane movements as seg_sum (a maximum does not
+// depend on the order).
+__device__ __forceinline__ float fmax_xor32(float v) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v),
+                                                    false, false);
+    return fmaxf(__uint_as_float(r[0]), __uint_as_float(r[1]));
+}
+
+__device__ __forceinline__ float fmax_xor16(float v) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v),
+                                                    false, false);
+    return fmaxf(__uint_as_float(r[0]), __uint_as_float(r[1]));
+}
+
+__device__ __forceinline__ double fmax_xor32(double v) {
+    const long long u = __double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)u, (unsigned)u, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(u >> 32), (unsigned)(u >> 32),
+                                                     false, false);
+    const double a = __longlong_as_double(((long long)hi[0] << 32) | (long long)lo[0]);
+    const double b = __longlong_as_double(((long long)hi[1] << 32) | (long long)lo[1]);
+    return fmax(a, b);
+}
+
+__device__ __forceinline__ double fmax_xor16(double v) {
+    const long long u = __double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)u, (unsigned)u, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(u >> 32), (unsigned)(u >> 32),
+                                                     false, false);
+    const double a = __longlong_as_double(((long long)hi[0] << 32) | (long long)lo[0]);
+    const double b = __longlong_as_double(((long long)hi[1] << 32) | (long long)lo[1]);
+    return fmax(a, b);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_move_f(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+template <int WIDTH>
+__device__ __forceinline__ float seg_max(float v) {
+    static_assert(WIDTH == 8 || WIDTH == 16 || WIDTH == 32 || WIDTH == 64,
+                  "segment of 8, 16, 32 or 64 lanes");
+    if constexpr (WIDTH == 8) {
+        v = fmaxf(v, dpp_move_f<0x141>(v));  // row_half_mirror
+        v = fmaxf(v, dpp_move_f<0x4E>(v));   // quad_perm [2,3,0,1]
+        v = fmaxf(v, dpp_move_f<0xB1>(v));   // quad_perm [1,0,3,2]
+        return v;
+    }
+    if constexpr (WIDTH >= 64) v = fmax_xor32(v);
+    if constexpr (WIDTH >= 32) v = fmax_xor16(v);
+    v = fmaxf(v, dpp_move_f<0x128>(v));  // row_ror:8
+    v = fmaxf(v, dpp_move_f<0x124>(v));  // row_ror:4
+    v = fmaxf(v, dpp_move_f<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = fmaxf(v, dpp_move_f<0xB1>(v));   // quad_perm [1,0,3,2]
+    return v;
+}
+
+template <int WIDTH>
+__device__ __forceinline__ double seg_max(double v) {
+    static_assert(WIDTH == 8 || WIDTH == 16 || WIDTH == 32 || WIDTH == 64,
+                  "segment of 8, 16, 32 or 64 lanes");
+    if constexpr (WIDTH == 8) {
+        v = fmax(v, dpp_move<0x141>(v));
+        v = fmax(v, dpp_move<0x4E>(v));
+        v = fmax(v, dpp_move<0xB1>(v));
+        return v;
+    }
+    if constexpr (WIDTH >= 64) v = fmax_xor32(v);
+    if constexpr (WIDTH >= 32) v = fmax_xor16(v);
+    v = fmax(v, dpp_move<0x128>(v));
+    v = fmax(v, dpp_move<0x124>(v));
+    v = fmax(v, dpp_move<0x4E>(v));
+    v = fmax(v, dpp_move<0xB1>(v));
+    return v;
+}
+
 // Fixed-order block-wide sum of one double per thread (kBlock threads).
 // `scratch` holds kWavesPerBlock doubles.  Result valid in every thread.
 __device__ __forceinline__ double block_sum(double v, double *scratch) {

@@ -27,6 +27,14 @@ struct QrPassArgs {
     // previous pass, zout written by this one (see drot_qr_tile.h)
     const unsigned char *zin;
     unsigned char *zout;
+    // screening of flagged segments (null: off; needs the flags and the
+    // branch-free tile): headrooms [n_rt][n_cols], one G1 per tile
+    // [n_rt * n_ct] (tile rt + ct * n_rt), row potentials of the previous
+    // pass (read) and of this one (written)
+    float *seg_e;
+    double *g1;
+    const float *f1_prev;
+    float *f1_out;
     IterCtl ctl;
 };
 
@@ -77,6 +85,10 @@ struct InitArgs {
     float v_phi1, v_phi2, v_a, v_b, v_alpha;
     unsigned char *zf;  // QR zero-segment flags, cleared here (null: none)
     size_t zf_bytes;
+    float *seg_e;       // QR screening headrooms, set to +inf here (null: none)
+    size_t seg_e_count;
+    double *g1;         // QR screening drift bounds, zeroed here
+    size_t g1_count;
 };
 
 struct QrTile {
@@ -170,6 +182,14 @@ struct QrBatchArgs {
     // iteration k reads buffer (k + 1) & 1 and writes buffer k & 1
     unsigned char *zf;
     long long zf_len;
+    // screening (null: off): headrooms packed like the flags ([n_rt][n_cols]
+    // floats per problem at zf_off), one G1 per tile (at tile_off), and two
+    // buffers of row potentials (packed like p, row_len floats each; the pass
+    // of iteration k reads buffer (k + 1) & 1 and writes buffer k & 1)
+    float *seg_e;
+    double *g1;
+    float *f1pb;
+    long long row_len;
 };
 
 // Batched tiles are 64 rows by 256 columns: 16 lanes per column holding 4

@@ -90,6 +90,11 @@ __global__ __launch_bounds__(kBlock) void init_kernel(InitArgs A) {
     // no segment starts out flagged: the first pass reads all of X
     if (A.zf)
         for (size_t i = tid; i < A.zf_bytes; i += nthr) A.zf[i] = 0;
+    // no headroom known yet: nothing is screened before its C has been read
+    if (A.seg_e) {
+        for (size_t i = tid; i < A.seg_e_count; i += nthr) A.seg_e[i] = INFINITY;
+        for (size_t i = tid; i < A.g1_count; i += nthr) A.g1[i] = 0.0;
+    }
     if (tid == 0) {
         SolverState s;
         s.done = 0;

@@ -15,8 +15,9 @@
 namespace drot {
 
 // ST: store policy of X (see Io); ST == 2 needs tc * n_rows * 4 < 2^31.
-// ZS: zero-segment flags (A.zin / A.zout) in use.
-template <int VEC, int LPC, bool EVEN, int ST, int U, bool ZS>
+// ZS: zero-segment flags (A.zin / A.zout) in use; SC: screening as well
+// (A.seg_e, A.g1, A.f1_prev / A.f1_out; branch-free tiles only).
+template <int VEC, int LPC, bool EVEN, int ST, int U, bool ZS, bool SC>
 __global__ __launch_bounds__(kBlock) void qr_pass_kernel(QrPassArgs A) {
     constexpr int CPW = kWave / LPC;
     constexpr int TR = LPC * VEC;
@@ -56,7 +57,11 @@ __global__ __launch_bounds__(kBlock) void qr_pass_kernel(QrPassArgs A) {
     J.kappa = A.kappa;
     J.zin = A.zin;
     J.zout = A.zout;
-    qr_tile<VEC, LPC, EVEN, ST, U, ZS>(J, A.ctl, blockIdx.x == 0, s_row, s_red);
+    J.seg_e = A.seg_e;
+    J.g1 = A.g1 ? A.g1 + tile : nullptr;
+    J.f1_prev = A.f1_prev;
+    J.f1_out = A.f1_out;
+    qr_tile<VEC, LPC, EVEN, ST, U, ZS, SC>(J, A.ctl, blockIdx.x == 0, s_row, s_red);
     if (A.ctl.tspan) record_span(A.ctl.tspan, t0);
 }
 
@@ -95,22 +100,22 @@ static int grid_capacity(const void *k, std::atomic<int> *cache) {
     return cap;
 }
 
-template <bool EVEN, int VEC, int LPC, int ST, int U, bool ZS>
+template <bool EVEN, int VEC, int LPC, int ST, int U, bool ZS, bool SC>
 static int pass_capacity() {
     static std::atomic<int> cache[kMaxDevices];  // zero-initialised (static storage)
     return grid_capacity(
-        reinterpret_cast<const void *>(qr_pass_kernel<VEC, LPC, EVEN, ST, U, ZS>), cache);
+        reinterpret_cast<const void *>(qr_pass_kernel<VEC, LPC, EVEN, ST, U, ZS, SC>), cache);
 }
 
-template <int VEC, int LPC, int ST, int U, bool ZS>
+template <int VEC, int LPC, int ST, int U, bool ZS, bool SC>
 static void launch_pass_z(bool even, int grid, const QrPassArgs &a, hipStream_t s,
                           hipEvent_t e0, hipEvent_t e1) {
-    auto k = even ? qr_pass_kernel<VEC, LPC, true, ST, U, ZS>
-                  : qr_pass_kernel<VEC, LPC, false, ST, U, ZS>;
+    auto k = even ? qr_pass_kernel<VEC, LPC, true, ST, U, ZS, SC>
+                  : qr_pass_kernel<VEC, LPC, false, ST, U, ZS, SC>;
     // ranking waves by remaining work pays only when the whole grid is resident
     // (one round of blocks); with later rounds it costs a few percent
-    const int cap = even ? pass_capacity<true, VEC, LPC, ST, U, ZS>()
-                         : pass_capacity<false, VEC, LPC, ST, U, ZS>();
+    const int cap = even ? pass_capacity<true, VEC, LPC, ST, U, ZS, SC>()
+                         : pass_capacity<false, VEC, LPC, ST, U, ZS, SC>();
     QrPassArgs b = a;
     b.prio = grid <= cap ? 1 : 0;
     if (e0 && e1)  // timing: events bracket the kernel's own execution
@@ -122,8 +127,12 @@ static void launch_pass_z(bool even, int grid, const QrPassArgs &a, hipStream_t 
 template <int VEC, int LPC, int ST, int U>
 static void launch_pass_t(bool even, int grid, const QrPassArgs &a, hipStream_t s,
                           hipEvent_t e0, hipEvent_t e1) {
-    if (a.zin && a.zout) launch_pass_z<VEC, LPC, ST, U, true>(even, grid, a, s, e0, e1);
-    else launch_pass_z<VEC, LPC, ST, U, false>(even, grid, a, s, e0, e1);
+    if (a.zin && a.zout && a.seg_e)
+        launch_pass_z<VEC, LPC, ST, U, true, true>(even, grid, a, s, e0, e1);
+    else if (a.zin && a.zout)
+        launch_pass_z<VEC, LPC, ST, U, true, false>(even, grid, a, s, e0, e1);
+    else
+        launch_pass_z<VEC, LPC, ST, U, false, false>(even, grid, a, s, e0, e1);
 }
 
 template <int VEC, int LPC>

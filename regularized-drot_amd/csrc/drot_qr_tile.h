@@ -26,6 +26,19 @@
 // forms fmaf(-s, c, +0).  Every element sees the same operands as without the
 // flags, so results keep their bits, and a settled segment moves 4 bytes per
 // element (C) instead of 12 (even) or 8 (odd).
+//
+// Screening (SC, with the flags).  A flagged segment also skips its C load,
+// and with it every load, when a bound proves that all its X' stay +0.  For
+// element i of segment (tile rows, column j), y = f1_i + f2_j - s c_ij up to
+// the roundings of the pass; the segment keeps a headroom E >= max_i (f1_i -
+// s c_ij) - G1 from the last pass that read its C, where G1 is the tile's
+// running sum of its rows' largest rise of f1 from one pass to the next, so
+// that E + G1(now) bounds max_i (f1_i - s c_ij) at any later pass.  A flagged
+// segment is skipped when E + G1 + f2_j lies below -2^-21 (max|f1| + |f2_j|),
+// which covers the roundings of y (at most three, each of 2^-24 of the
+// largest operand): every y is then negative, X' = +0 as the full pass would
+// give, and the flag stays set.  The first batch of a tile is issued before
+// the prologue and is not screened.
 #pragma once
 
 #include <type_traits>
@@ -36,6 +49,7 @@ namespace drot {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kBufDword3 = 0x00020000;  // raw buffer, 32-bit data format
 constexpr int kPolicySc1 = 16;          // cache-policy operand: sc1
@@ -105,6 +119,13 @@ struct QrTileJob {
     // (needs nstep <= LPC: lane rl of a column group loads the flag of step rl)
     const unsigned char *zin;
     unsigned char *zout;
+    // screening (SC): the problem's [n_rt][n_cols] headrooms E, this tile's G1,
+    // and the problem's row potentials of the previous pass (read) and of this
+    // one (written by the tiles of column tile 0)
+    float *seg_e;
+    double *g1;
+    const float *f1_prev;
+    float *f1_out;
 };
 
 // y / kappa, correctly rounded, from the correctly rounded reciprocal
@@ -308,7 +329,7 @@ __device__ __forceinline__ bool div_rn_slow(float a, float q) {
     return a != 0.0f && !(fabsf(q) >= 0x1p-90f && fabsf(q) <= 0x1p+120f);
 }
 
-template <int VEC, int LPC, bool EVEN, int ST, int U, bool ZS>
+template <int VEC, int LPC, bool EVEN, int ST, int U, bool ZS, bool SC>
 __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &ctl,
                                              bool leader_block, double (*s_row)[LPC * VEC],
                                              double *s_red) {
@@ -364,11 +385,32 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
     using ZMask = std::conditional_t<(LPC > 32), unsigned long long, unsigned>;
     constexpr int kZBits = 8 * (int)sizeof(ZMask);
     ZMask zmask = 0;
+    bool zb = false;
+    // screening data: lane rl holds step rl's headroom and column potential
+    float e_rel = INFINITY, h_scr = 0.0f;
     if (ZS) {
         const int jz = tile_col0 + jt0 + rl * kColStride;
-        const bool zb = rl < nstep && jz < M && J.zin[(size_t)J.rt * M + jz] != 0;
+        const bool zok = rl < nstep && jz < M;
+        zb = zok && J.zin[(size_t)J.rt * M + jz] != 0;
+        if constexpr (SC) {
+            e_rel = zok ? J.seg_e[(size_t)J.rt * M + jz] : INFINITY;
+            h_scr = zok ? hat2[jz] : 0.0f;
+        }
         zmask = (ZMask)group_bits<LPC>(__builtin_amdgcn_ballot_w64(zb), cs);
     }
+    static_assert(!SC || ZS, "screening needs the zero-segment flags");
+    double g1_old = 0.0;
+    float f1p[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; e++) f1p[e] = 0.0f;
+    if constexpr (SC) {
+        g1_old = *J.g1;
+        if (ctl.k > 0) {
+#pragma unroll
+            for (int e = 0; e < VEC; e++) f1p[e] = rows[e] < N ? J.f1_prev[rows[e]] : 0.0f;
+        }
+    }
+    ZMask smask = 0;  // screened steps of this column group (after the prologue)
 
     float xa[U][VEC], ca[U][VEC], ha[U];
     float xb[U][VEC], cb[U][VEC], hb[U];
@@ -385,15 +427,18 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
                 hr, (int)(h_lane + (unsigned)(t * kColStride) * 4u), 0, 0));
             if constexpr (ZS) {
                 const bool z = (zmask >> (t & (kZBits - 1))) & 1;
+                const bool sk = SC && ((smask >> (t & (kZBits - 1))) & 1);
                 // a flagged segment: the even pass reads no X (+0), the odd
-                // pass reads C (and X as +0) to form what the even pass left
-                unsigned xoff[NCH], coff[NCH];
+                // pass reads C (and X as +0) to form what the even pass left;
+                // a screened one reads nothing
+                unsigned xoff[NCH], coff[NCH], eoff[NCH];
 #pragma unroll
                 for (int h = 0; h < NCH; h++) {
                     xoff[h] = z ? kOob : off[h];
-                    coff[h] = z ? off[h] : kOob;
+                    coff[h] = (z && !sk) ? off[h] : kOob;
+                    eoff[h] = sk ? kOob : off[h];
                 }
-                if (EVEN) IoV::load(cr, off, cv[u]);
+                if (EVEN) IoV::load(cr, eoff, cv[u]);
                 IoV::load(xr, xoff, xv[u]);
                 if (!EVEN) IoV::load(cr, coff, cv[u]);
             } else {
@@ -434,12 +479,73 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
     for (int e = 0; e < VEC; e++) racc[e] = 0.0;
     double oacc = 0.0;
 
+    // screening: G1 += the largest rise of the tile's f1 since the previous
+    // pass (exact differences in fp64; every column group holds the tile's
+    // rows, so every lane ends with the same sum), then the screen of step rl
+    // of this lane's column group
+    double g1_new = 0.0;
+    if constexpr (SC) {
+        // (the rise of a row is exact in fp64 and rounded up to fp32)
+        float rise = -INFINITY, f1abs = 0.0f;
+#pragma unroll
+        for (int e = 0; e < VEC; e++) {
+            if (rows[e] < N) {
+                rise = fmaxf(rise, __double2float_ru((double)f1[e] - (double)f1p[e]));
+                f1abs = fmaxf(f1abs, fabsf(f1[e]));
+            }
+        }
+        rise = seg_max<LPC>(rise);
+        f1abs = seg_max<LPC>(f1abs);
+        g1_new = ctl.k > 0 ? g1_old + (double)rise : 0.0;
+        if (threadIdx.x == 0) *J.g1 = g1_new;
+        if (J.ct == 0 && w == 0 && cs == 0) {
+#pragma unroll
+            for (int e = 0; e < VEC; e++)
+                if (rows[e] < N) J.f1_out[rows[e]] = f1[e];
+        }
+        // f2 of step rl within an ulp or two of the pass's own (the bound's
+        // slack covers it); an unset headroom (+inf) never screens
+        const float f2s = explicit_phi ? h_scr : (h_scr + gamma) / fN;
+        const double bnd = (double)e_rel + g1_new + (double)f2s;
+        const double slack = 0x1p-21 * ((double)f1abs + fabs((double)f2s)) +
+                             0x1p-40 * (fabs((double)e_rel) + fabs(g1_new)) + 0x1p-120;
+        // a step past the problem's columns counts as screened as well: its
+        // loads read nothing and its stores are dropped either way
+        const bool past = rl < nstep && tile_col0 + jt0 + rl * kColStride >= M;
+        const bool sk = (zb && ctl.k > 0 && bnd + slack < 0.0) || past;
+        smask = (ZMask)group_bits<LPC>(__builtin_amdgcn_ballot_w64(sk), cs);
+    }
+    // headroom of a segment whose C this pass reads, stored by lane 0 of the
+    // column group: max_i RN(f1_i - s c_ij) = R bounds the exact maximum by
+    // R + 2^-23 |R|, kept relative to G1 and rounded up to fp32.  Refreshed on
+    // every eighth pass (an even one, which reads the C of every segment it
+    // does not screen); a headroom keeps its validity as G1 grows, and with
+    // this interval the simulated screen still certifies 99.8% of the flagged
+    // segments at 1000^2, as it does when refreshing every pass.
+    const __amdgpu_buffer_rsrc_t er =
+        SC ? buffer_over(J.seg_e + part_base, (unsigned)tile_cols * 4u) : pr;
+    const unsigned e_lane = rl == 0 ? (unsigned)jt0 * 4u : kOob;
+    const bool e_refresh = (ctl.k & 7) == 0;
+
     auto run_batch = [&](int t0, float (&xv)[U][VEC], float (&cv)[U][VEC],
                          float (&hv)[U]) __attribute__((always_inline)) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int t = t0 + u;
             if (t >= nstep) break;  // wave-uniform
+            if constexpr (SC) {
+                // every column group of the wave screened at this step: the
+                // step's results are known without arithmetic (X' = +0: the
+                // flag stays set, the column partial is +0, X is not stored,
+                // the row and objective sums gain +0) and it loaded nothing
+                if (__builtin_amdgcn_ballot_w64(((smask >> (t & (kZBits - 1))) & 1) == 0) == 0ull) {
+                    __builtin_amdgcn_raw_buffer_store_b8(
+                        (unsigned char)1, zr, (int)(z_lane + (unsigned)(t * kColStride)), 0, 0);
+                    __builtin_amdgcn_raw_buffer_store_b64(
+                        (u32x2)(0u), pr, (int)(p_lane + (unsigned)(t * kColStride) * 8u), 0, 0);
+                    continue;
+                }
+            }
             // phi2 = (hat + gamma) / n_rows; at k == 0 the array value itself,
             // as (value + 0) / 1 (a zero's sign aside, which no X' sees)
             const float num = hv[u] + g2;
@@ -450,7 +556,10 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
             if (__builtin_amdgcn_ballot_w64(num != 0.0f) &
                 __builtin_amdgcn_ballot_w64(qb - kQLo > kQHi - kQLo))
                 if (div_rn_slow(num, f2)) f2 = num / d2;
-            f2 = t * kColStride < col_lim ? f2 : -INFINITY;
+            // a screened step computes as a column past the problem: every
+            // y is -inf, X' = +0, nothing is stored and the flag stays set
+            const bool sk = SC && ((smask >> (t & (kZBits - 1))) & 1);
+            f2 = (t * kColStride < col_lim && !sk) ? f2 : -INFINITY;
             if (EVEN && pad) {
 #pragma unroll
                 for (int e = 0; e < VEC; e++)
@@ -513,9 +622,19 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
 #pragma unroll
             for (int e = 0; e < VEC; e++) xo[e] = EVEN ? __builtin_fmaf(-s, cv[u][e], xq[e]) : xq[e];
             IoV::template store<kPol>(xr, st_off, xo);
+            if (SC && EVEN && e_refresh) {  // k % 8 == 0: an even pass, C read unless screened
+                const bool upd = !sk;
+                float r = -INFINITY;
+#pragma unroll
+                for (int e = 0; e < VEC; e++) r = fmaxf(r, __builtin_fmaf(-s, cv[u][e], f1[e]));
+                r = seg_max<LPC>(r);
+                const double ev = (double)r + 0x1p-23 * fabs((double)r) - g1_new;
+                __builtin_amdgcn_raw_buffer_store_b32(
+                    __float_as_uint(__double2float_ru(ev)), er,
+                    (int)(upd ? e_lane + (unsigned)(t * kColStride) * 4u : kOob), 0, 0);
+            }
             csum = seg_sum<LPC>(csum);
             const unsigned long long cb64 = (unsigned long long)__double_as_longlong(csum);
-            typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
             u32x2 cw;
             cw.x = (unsigned)cb64;
             cw.y = (unsigned)(cb64 >> 32);
@@ -746,17 +865,19 @@ __device__ __forceinline__ void qr_tile_generic(const QrTileJob &J, const IterCt
     tile_partials_out<VEC, LPC, EVEN>(J, racc, oacc, s_row);
 }
 
-template <int VEC, int LPC, bool EVEN, int ST, int U, bool ZS>
+// SC (screening) runs in the branch-free form only; the host enables it for
+// shapes that satisfy qr_lean_ok.
+template <int VEC, int LPC, bool EVEN, int ST, int U, bool ZS, bool SC>
 __device__ __forceinline__ void qr_tile(const QrTileJob &J, const IterCtl &ctl, bool leader_block,
                                         double (*s_row)[LPC * VEC], double *s_red) {
-    if constexpr (VEC == 8) {
+    if constexpr (VEC == 8 || SC) {
         // two-chunk slices exist in the lean form only; the host gives such
         // tiles to shapes that satisfy qr_lean_ok (rows a multiple of 4)
         if (qr_lean_ok<VEC>(J))
-            qr_tile_lean<VEC, LPC, EVEN, ST, U, ZS>(J, ctl, leader_block, s_row, s_red);
+            qr_tile_lean<VEC, LPC, EVEN, ST, U, ZS, SC>(J, ctl, leader_block, s_row, s_red);
     } else {
         if (qr_lean_ok<VEC>(J)) {  // wave-uniform
-            qr_tile_lean<VEC, LPC, EVEN, ST, U, ZS>(J, ctl, leader_block, s_row, s_red);
+            qr_tile_lean<VEC, LPC, EVEN, ST, U, ZS, false>(J, ctl, leader_block, s_row, s_red);
             return;
         }
         qr_tile_generic<VEC, LPC, EVEN, ST, U, ZS>(J, ctl, leader_block, s_row, s_red);

@@ -73,6 +73,10 @@ struct Plan {
     // QR zero-segment flags: two buffers of zf_bytes ([n_rt][n_cols]) at
     // off_zf, or none (zf_bytes == 0)
     size_t off_zf = 0, zf_bytes = 0;
+    // QR screening of flagged segments (drot_qr_tile.h): headrooms
+    // [n_rt][n_cols] (fp32), one G1 per tile (fp64), two row-potential buffers
+    int scr = 0;
+    size_t off_seg_e = 0, off_g1 = 0, off_f1p = 0;
 };
 
 // Whether a supported tile may run a shape.  16-byte tiles need 16-byte
@@ -124,6 +128,9 @@ void layout(Plan &P) {
     P.off_bounds = take((size_t)(P.n_groups + 1) * 4);
     P.off_rowgroup = take(P.kind == kGL ? N * 4 : 0);
     P.off_zf = take(2 * P.zf_bytes);
+    P.off_seg_e = take(P.scr ? (size_t)P.n_rt * M * 4 : 0);
+    P.off_g1 = take(P.scr ? (size_t)P.n_rt * P.n_ct * 8 : 0);
+    P.off_f1p = take(P.scr ? 2 * N * 4 : 0);
     const bool split = P.kind == kGL && P.gt.split;
     P.off_pnorm = take(split ? (size_t)P.n_seg * M * 8 : 0);
     P.off_gseg = take(split ? (size_t)(P.n_groups + 1) * 4 : 0);
@@ -177,6 +184,13 @@ int gl_segments(const std::vector<int> &hb, int chunk, std::vector<int> *gseg,
 // to 1e-4 with / without them on MI355X: 1000^2 to 2048^2 1.01-1.03x,
 // 2500^2 0.99x, 3000^2 0.94x, 10000^2 0.69x per iteration.
 constexpr long long kZeroSkipMinElems = 6000000;
+// Screening of flagged segments (with the flags) from this many elements up,
+// for single problems and for a batch's total: it pays once C and X stream
+// from HBM rather than the 256 MB last-level cache.  MI355X, time to 1e-4 with
+// and without: 10000^2 269 vs 379 ms, 8000^2 250 vs 266, 5000^2 115 vs 106,
+// 3000^2 51 vs 45; mixed 256-problem batch forward 734 vs 814 ms, 64 x 512^2
+// forward 45 vs 42 ms.
+constexpr long long kScreenMinElems = 50000000;
 
 // Plan of a QR solve with a given pass tile; nstep <= 0 picks the column steps.
 Plan plan_qr_tile(int n_rows, int n_cols, QrTile t, int nstep) {
@@ -212,6 +226,15 @@ Plan plan_qr_tile(int n_rows, int n_cols, QrTile t, int nstep) {
     const bool want = (zs && zs[0] == '1') || (big && !(zs && zs[0] == '0'));
     const bool zskip = nstep <= P.qt.lpc && want;
     P.zf_bytes = zskip ? (size_t)P.n_rt * (size_t)n_cols : 0;
+    // screening of flagged segments, with the flags, in the branch-free tile
+    // (rows in whole 16-byte chunks for vec 4 and 8).  DROT_QR_SCREEN=1 / 0
+    // turns it on / off wherever the flags are on.
+    const char *sc = getenv("DROT_QR_SCREEN");
+    const bool lean = (P.qt.vec == 1 || n_rows % 4 == 0) &&
+                      (long long)(P.tc + 64) * n_rows * 4 < (1LL << 31);
+    const bool want_sc = (sc && sc[0] == '1') ||
+                         ((long long)n_rows * n_cols >= kScreenMinElems && !(sc && sc[0] == '0'));
+    P.scr = (zskip && lean && want_sc) ? 1 : 0;
     layout(P);
     return P;
 }
@@ -496,6 +519,9 @@ struct DevPtrs {
     int *bounds;
     int *row_group;
     unsigned char *zf;  // null when the plan has no zero-segment flags
+    float *seg_e;       // screening (null: off)
+    double *g1;
+    float *f1p;
 };
 
 DevPtrs carve(const Plan &P, void *ws) {
@@ -515,6 +541,9 @@ DevPtrs carve(const Plan &P, void *ws) {
     d.bounds = reinterpret_cast<int *>(base + P.off_bounds);
     d.row_group = reinterpret_cast<int *>(base + P.off_rowgroup);
     d.zf = P.zf_bytes ? reinterpret_cast<unsigned char *>(base + P.off_zf) : nullptr;
+    d.seg_e = P.scr ? reinterpret_cast<float *>(base + P.off_seg_e) : nullptr;
+    d.g1 = P.scr ? reinterpret_cast<double *>(base + P.off_g1) : nullptr;
+    d.f1p = P.scr ? reinterpret_cast<float *>(base + P.off_f1p) : nullptr;
     return d;
 }
 
@@ -580,6 +609,10 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
     ia.warm = in.warm;
     ia.zf = d.zf;
     ia.zf_bytes = 2 * P.zf_bytes;
+    ia.seg_e = d.seg_e;
+    ia.seg_e_count = P.scr ? (size_t)P.n_rt * (size_t)P.n_cols : 0;
+    ia.g1 = d.g1;
+    ia.g1_count = P.scr ? (size_t)P.n_rt * (size_t)P.n_ct : 0;
     warm_constants(P.n_rows, P.n_cols, in.step, &ia);
     DROT_HIP_CHECK(launch_init(ia, stream));
 
@@ -764,6 +797,12 @@ drot_status run(const Plan &P, const SolveIn &in, void *ws, hipStream_t stream,
                 // pass k reads the flags pass k-1 wrote and writes the others
                 qa.zin = d.zf ? d.zf + (size_t)((k + 1) & 1) * P.zf_bytes : nullptr;
                 qa.zout = d.zf ? d.zf + (size_t)(k & 1) * P.zf_bytes : nullptr;
+                if (d.seg_e) {
+                    qa.seg_e = d.seg_e;
+                    qa.g1 = d.g1;
+                    qa.f1_prev = d.f1p + (size_t)((k + 1) & 1) * P.n_rows;
+                    qa.f1_out = d.f1p + (size_t)(k & 1) * P.n_rows;
+                }
                 DROT_HIP_CHECK(launch_qr_pass(P.qt, (k % 2) == 0, P.grid_pass, qa, stream, e0, e1));
             } else {
                 ga.ctl.k = k;
@@ -1275,6 +1314,10 @@ struct BatchPlan {
     size_t off_probs = 0, off_tiles = 0, off_fins = 0, total = 0;
     long long zf_len = 0;  // bytes per zero-segment flag buffer (0: no flags)
     size_t off_zf = 0;
+    // screening of flagged segments (with the flags): zf_len fp32 headrooms,
+    // one fp64 G1 per tile, two row-potential buffers of row_len floats
+    int scr = 0;
+    size_t off_seg_e = 0, off_g1 = 0, off_f1p = 0;
 };
 
 drot_status plan_batch(int n, const drot_batch_problem *in, BatchPlan *out) {
@@ -1376,6 +1419,16 @@ drot_status plan_batch(int n, const drot_batch_problem *in, BatchPlan *out) {
     const char *zs = getenv("DROT_QR_ZSKIP");
     B.zf_len = (zs && zs[0] == '0') ? 0 : zf_off;
     B.off_zf = take(2 * (size_t)B.zf_len);
+    // screening with the flags, for batches whose matrices add up to
+    // kScreenMinElems elements or more; DROT_QR_SCREEN=1 / 0 turns it on / off
+    const char *sc = getenv("DROT_QR_SCREEN");
+    long long elems = 0;
+    for (const auto &P : B.probs) elems += (long long)P.n_rows * P.n_cols;
+    const bool want_sc = (sc && sc[0] == '1') || (elems >= kScreenMinElems && !(sc && sc[0] == '0'));
+    B.scr = (B.zf_len > 0 && want_sc) ? 1 : 0;
+    B.off_seg_e = take(B.scr ? 4 * (size_t)B.zf_len : 0);
+    B.off_g1 = take(B.scr ? 8 * (size_t)tile_off : 0);
+    B.off_f1p = take(B.scr ? 2 * 4 * (size_t)B.row_len : 0);
     B.total = off;
     *out = std::move(B);
     return DROT_OK;
@@ -1440,6 +1493,10 @@ drot_status run_batch(const BatchPlan &B, const float *c, const float *p, const 
     A.n_probs = n;
     A.zf = B.zf_len ? reinterpret_cast<unsigned char *>(base + B.off_zf) : nullptr;
     A.zf_len = B.zf_len;
+    A.seg_e = B.scr ? reinterpret_cast<float *>(base + B.off_seg_e) : nullptr;
+    A.g1 = B.scr ? reinterpret_cast<double *>(base + B.off_g1) : nullptr;
+    A.f1pb = B.scr ? reinterpret_cast<float *>(base + B.off_f1p) : nullptr;
+    A.row_len = B.row_len;
 
     const size_t list_bytes =
         sizeof(BatchTile) * B.tiles.size() + sizeof(BatchFin) * B.fins.size();

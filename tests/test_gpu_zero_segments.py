@@ -13,6 +13,11 @@ off, so the same solve with and without them must give identical bits:
     64 steps on 64-lane tiles (flag masks wider than 32 bits).
 The plans here are sparse from the first tens of iterations, so most segments
 take the flagged path.
+
+Screening (DROT_QR_SCREEN=1, branch-free tiles): a flagged segment whose
+headroom bound proves all its X' stay +0 loads nothing at all.  Every case runs
+with the flags and screening, with the flags alone, and without either: all
+three give the same bits.
 """
 import os
 
@@ -24,15 +29,19 @@ from reg_drot import problems
 pytestmark = pytest.mark.gpu
 
 
-def _solve(c, p, q, prm, device, zskip, tile=None):
+def _solve(c, p, q, prm, device, zskip, tile=None, screen=None):
     from test_gpu_qr import gpu_qr
-    old = {k: os.environ.get(k) for k in ("DROT_QR_ZSKIP", "DROT_QR_TILE")}
+    old = {k: os.environ.get(k) for k in ("DROT_QR_ZSKIP", "DROT_QR_TILE", "DROT_QR_SCREEN")}
     try:
         os.environ["DROT_QR_ZSKIP"] = "1" if zskip else "0"
         if tile:
             os.environ["DROT_QR_TILE"] = tile
         else:
             os.environ.pop("DROT_QR_TILE", None)
+        if screen is None:
+            os.environ.pop("DROT_QR_SCREEN", None)
+        else:
+            os.environ["DROT_QR_SCREEN"] = "1" if screen else "0"
         return gpu_qr(c, p, q, prm, device)
     finally:
         for k, v in old.items():
@@ -40,6 +49,17 @@ def _solve(c, p, q, prm, device, zskip, tile=None):
                 os.environ.pop(k, None)
             else:
                 os.environ[k] = v
+
+
+def _flags_keep_bits(c, p, q, prm, device, tile=None):
+    """The solve with flags and screening, with flags alone, and without
+    either; returns the first after checking all three agree bit for bit."""
+    off = _solve(c, p, q, prm, device, False, tile)
+    flags = _solve(c, p, q, prm, device, True, tile, screen=False)
+    screened = _solve(c, p, q, prm, device, True, tile, screen=True)
+    _same(flags, off)
+    _same(screened, off)
+    return screened
 
 
 def _same(a, b):
@@ -61,11 +81,9 @@ def test_flags_keep_bits(device, n_rows, n_cols, iters, eps):
     p, q = problems.uniform_marginals(n_rows, n_cols)
     prm = _abi.qr_params_from_rho(2.0, 1e-3, n_rows, n_cols, iters, eps)
     print(f"[zero segments] {n_rows}x{n_cols} iters={iters} eps={eps}", flush=True)
-    on = _solve(c, p, q, prm, device, True)
-    off = _solve(c, p, q, prm, device, False)
+    on = _flags_keep_bits(c, p, q, prm, device)
     if eps > 0:
         assert on[1].converged == 1
-    _same(on, off)
     # the plan is sparse: most of it is zero once converged (up to the residue,
     # at most half an ulp of sC, that fix_x leaves where X' = 0)
     if eps > 0:
@@ -84,7 +102,7 @@ def test_flags_keep_bits_short_runs_and_cold_start(device, iters, warm):
     prm = _abi.qr_params_from_rho(2.0, 1e-3, n_rows, n_cols, iters, 1e-4 if iters > 100 else 0.0)
     prm.warm_start = warm
     print(f"[zero segments] {n_rows}x{n_cols} iters={iters} warm={warm}", flush=True)
-    _same(_solve(c, p, q, prm, device, True), _solve(c, p, q, prm, device, False))
+    _flags_keep_bits(c, p, q, prm, device)
 
 
 @pytest.mark.parametrize("tile", ["4,16,7,2,2", "4,32,3", "4,64,2,2,4", "1,16,5",
@@ -98,8 +116,7 @@ def test_flags_keep_bits_every_tile(device, tile):
     for iters, eps in ((157, 0.0), (20000, 1e-4)):
         prm = _abi.qr_params_from_rho(2.0, 1e-3, n_rows, n_cols, iters, eps)
         print(f"[zero segments] tile {tile} iters={iters}", flush=True)
-        _same(_solve(c, p, q, prm, device, True, tile),
-              _solve(c, p, q, prm, device, False, tile))
+        _flags_keep_bits(c, p, q, prm, device, tile)
 
 
 @pytest.mark.parametrize("tile", ["8,8,8,2,1", "8,16,5,0,1", "8,8,3,0,1", "8,16,16,2,1"])
@@ -116,8 +133,7 @@ def test_flags_keep_bits_two_chunk_tiles(device, tile, monkeypatch):
     for iters, eps in ((157, 0.0), (20000, 1e-4)):
         prm = _abi.qr_params_from_rho(2.0, 1e-3, n_rows, n_cols, iters, eps)
         print(f"[zero segments] tile {tile} iters={iters}", flush=True)
-        _same(_solve(c, p, q, prm, device, True, tile),
-              _solve(c, p, q, prm, device, False, tile))
+        _flags_keep_bits(c, p, q, prm, device, tile)
 
 
 @pytest.mark.parametrize("max_iters,eps", [(20000, 1e-4), (151, 0.0)])
@@ -135,24 +151,27 @@ def test_batch_flags_keep_bits(device, max_iters, eps):
         ps.append(torch.from_numpy(p).to(device))
         qs.append(torch.from_numpy(q).to(device))
     out = {}
-    old = os.environ.get("DROT_QR_ZSKIP")
+    old = {k: os.environ.get(k) for k in ("DROT_QR_ZSKIP", "DROT_QR_SCREEN")}
     try:
-        for zs in ("1", "0"):
+        for zs, sc in (("1", "1"), ("1", "0"), ("0", "0")):
             os.environ["DROT_QR_ZSKIP"] = zs
-            print(f"[zero segments] batch of {len(shapes)} zskip={zs} max_iters={max_iters}",
-                  flush=True)
+            os.environ["DROT_QR_SCREEN"] = sc
+            print(f"[zero segments] batch of {len(shapes)} zskip={zs} screen={sc} "
+                  f"max_iters={max_iters}", flush=True)
             plans, infos = batch.quadratic_drot_batch(cs, ps, qs, 2.0, rws, max_iters, eps)
             torch.cuda.synchronize(device)
-            out[zs] = ([x.cpu().numpy() for x in plans], infos)
+            out[zs + sc] = ([x.cpu().numpy() for x in plans], infos)
     finally:
-        if old is None:
-            os.environ.pop("DROT_QR_ZSKIP", None)
-        else:
-            os.environ["DROT_QR_ZSKIP"] = old
-    for xa, xb, ia, ib in zip(out["1"][0], out["0"][0], out["1"][1], out["0"][1]):
-        assert ia["n_iter"] == ib["n_iter"] and ia["converged"] == ib["converged"]
-        assert ia["objective_f64"] == ib["objective_f64"]
-        assert np.array_equal(xa.view(np.uint32), xb.view(np.uint32))
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    for key in ("11", "10"):
+        for xa, xb, ia, ib in zip(out[key][0], out["00"][0], out[key][1], out["00"][1]):
+            assert ia["n_iter"] == ib["n_iter"] and ia["converged"] == ib["converged"]
+            assert ia["objective_f64"] == ib["objective_f64"]
+            assert np.array_equal(xa.view(np.uint32), xb.view(np.uint32))
 
 
 def test_flags_keep_bits_awkward_inputs(device):
@@ -173,5 +192,4 @@ def test_flags_keep_bits_awkward_inputs(device):
     for iters, eps in ((20000, 1e-4), (333, 0.0)):
         prm = _abi.qr_params_from_rho(2.0, 1e-3, n_rows, n_cols, iters, eps)
         print(f"[zero segments] awkward inputs iters={iters}", flush=True)
-        _same(_solve(c, p.astype(np.float32), q.astype(np.float32), prm, device, True),
-              _solve(c, p.astype(np.float32), q.astype(np.float32), prm, device, False))
+        _flags_keep_bits(c, p.astype(np.float32), q.astype(np.float32), prm, device)
