@@ -104,6 +104,10 @@ def parse():
                     help="run one GL-DROT solve of exactly this many iterations on the "
                          "domain-adaptation task (1500 x 1000, 2 groups, core lambda "
                          "r_weight 1e-3, seed 0) and exit (for rocprofv3 runs)")
+    ap.add_argument("--profile-batched", action="store_true",
+                    help="run the two batched forward+backward workloads at eps 1e-4 once "
+                         "each, batched and looped, printing a line after each, and exit "
+                         "(for rocprofv3 runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run plain device copies of one 3000x3000 fp32 matrix and exit "
                          "(reference reading for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -440,6 +444,10 @@ def main():
         return
     if os.path.exists(tune.DEFAULT_TABLE):
         tune.load(tune.DEFAULT_TABLE)
+    if args.profile_batched:
+        for workload in ("differentiable", "mixed"):
+            print(json.dumps({workload: batched_fwd_bwd(dev, workload, warm=False)}), flush=True)
+        return
     if args.profile_gl > 0:
         n_src, n_tgt = 1500, 1000
         cg = torch.from_numpy(problems.domain_adaptation(n_src, n_tgt, seed=0)[0]).to(dev)
