@@ -270,8 +270,12 @@ bool tuned_tile(int n_rows, int n_cols, bool v4, QrTile *t, int *nstep) {
 // tiles) and the fewest column steps (at most 8) whose grid fits one resident
 // round of the even pass (kResidentPass blocks: 5 per CU on 256 CUs), grown
 // while tiles hold under kSmallTile elements and the grid keeps kMinGrid
-// blocks; a grid beyond two rounds even at 8 steps (HBM-resident problems)
-// takes 32 lanes per column, 8 steps and plain stores.
+// blocks.  A grid beyond two rounds even at 8 steps takes the tuned 10000^2
+// tile when the rows allow two 16-byte chunks per lane: 8 rows per lane, 16
+// lanes per column, 16 column steps, write-through stores while C and X fit
+// the 256 MB last-level cache and plain ones beyond (MI355X, per iteration
+// against the 32-lane tile below: 5000^2 28.2 vs 34.0 us); otherwise 32 lanes
+// per column, 8 steps and plain stores.
 constexpr long long kResidentPass = 1280;
 constexpr long long kMinGrid = 512;
 constexpr int kSmallTile = 4096;
@@ -283,8 +287,14 @@ void builtin_tile16(int n_rows, int n_cols, QrTile *t, int *nstep) {
     int ns = 1;
     while (ns < 8 && grid(ns) > kResidentPass) ns++;
     if (grid(ns) > 2 * kResidentPass) {
-        *t = QrTile{4, 32, 0, 2};
-        *nstep = 8;
+        const long long cx_bytes = 8LL * n_rows * n_cols;  // C and X
+        if (n_rows % 4 == 0 && (256LL + 64) * n_rows * 4 < (1LL << 31)) {
+            *t = QrTile{8, 16, cx_bytes < (200LL << 20) ? 2 : 0, 1};
+            *nstep = 16;
+        } else {
+            *t = QrTile{4, 32, 0, 2};
+            *nstep = 8;
+        }
         return;
     }
     while (ns < 8 && tr * tc_min * ns < kSmallTile && grid(ns + 1) >= kMinGrid) ns++;

@@ -224,3 +224,41 @@ def test_zero_segment_flags_in_batch_workspace(monkeypatch):
     without = _abi.batch_workspace_size(probs)
     flags = 2 * sum(-(-r // 64) * c for r, c in zip(rows, cols))
     assert with_flags - without == -(-flags // 256) * 256
+
+
+def test_screening_buffers_in_qr_workspace(monkeypatch):
+    """QR solves from 50M elements (with the zero-segment flags, in the
+    branch-free tile) also reserve the screening state: an fp32 headroom per
+    flag, an fp64 bound per tile and two fp32 row buffers.  DROT_QR_SCREEN=1 /
+    0 turns it on / off wherever the flags are on."""
+    from reg_drot import _abi
+
+    def ws(n, sc):
+        if sc is None:
+            monkeypatch.delenv("DROT_QR_SCREEN", raising=False)
+        else:
+            monkeypatch.setenv("DROT_QR_SCREEN", sc)
+        return _abi.qr_workspace_size(n, n)
+
+    def a256(b):
+        return -(-b // 256) * 256
+
+    def screen_bytes(n):
+        # the size covers the plans with and without 16-byte row chunks; the
+        # one-float-per-lane plan (more row tiles) is the larger
+        t = _abi.qr_tile_for(n, n, False)
+        tr = t.vec * t.lanes_per_col
+        tc = t.col_steps * 4 * (64 // t.lanes_per_col)
+        n_rt, n_ct = -(-n // tr), -(-n // tc)
+        return a256(n_rt * n * 4) + a256(n_rt * n_ct * 8) + a256(2 * n * 4)
+
+    monkeypatch.delenv("DROT_QR_ZSKIP", raising=False)
+    n = 7072  # 50.01M elements
+    assert ws(n, None) - ws(n, "0") == screen_bytes(n)
+    assert ws(n, "1") == ws(n, None)
+    for n in (3000, 7000):  # below the threshold
+        assert ws(n, None) == ws(n, "0")
+        assert ws(n, "1") - ws(n, "0") == screen_bytes(n)
+    # no flags, no screening
+    monkeypatch.setenv("DROT_QR_ZSKIP", "0")
+    assert ws(7072, "1") == ws(7072, "0")
