@@ -533,7 +533,7 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
         for (int u = 0; u < U; u++) {
             const int t = t0 + u;
             if (t >= nstep) break;  // wave-uniform
-            if constexpr (SC) {
+            if constexpr (SC && U > 1) {
                 // every column group of the wave screened at this step: the
                 // step's results are known without arithmetic (X' = +0: the
                 // flag stays set, the column partial is +0, X is not stored,
@@ -643,14 +643,59 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
         }
     };
 
-    for (int t0 = 0; t0 < nstep; t0 += 2 * U) {
-        if (J.prio) set_issue_priority(nstep - t0, nstep);
-        if (t0 + U < nstep) load_batch(t0 + U, xb, cb, hb);
-        run_batch(t0, xa, ca, ha);
-        if (t0 + U >= nstep) break;
-        if (J.prio) set_issue_priority(nstep - t0 - U, nstep);
-        if (t0 + 2 * U < nstep) load_batch(t0 + 2 * U, xa, ca, ha);
-        run_batch(t0 + U, xb, cb, hb);
+    if constexpr (SC && U == 1) {
+        // steps at which every column group of the wave is screened
+        // (wave-uniform) have results known without arithmetic (X' = +0: the
+        // flag stays set, the column partial is +0, X is not stored, the row
+        // and objective sums gain +0) and load nothing; the loop runs over
+        // the other steps only.  (A skip branch inside the step instead kept
+        // 26 more registers live and cost a wave per SIMD.)
+        unsigned long long all_sk = ~0ull;
+#pragma unroll
+        for (int g = 0; g < CPW; g++) {
+            const unsigned long long m = (unsigned long long)smask;
+            const unsigned lo = __builtin_amdgcn_readlane((unsigned)m, g * LPC);
+            const unsigned hi = __builtin_amdgcn_readlane((unsigned)(m >> 32), g * LPC);
+            all_sk &= ((unsigned long long)hi << 32) | lo;
+        }
+        const unsigned long long steps = nstep >= 64 ? ~0ull : ((1ull << nstep) - 1ull);
+        unsigned long long live = steps & ~all_sk;
+        for (unsigned long long m = steps & all_sk; m; m &= m - 1ull) {
+            const int t = __builtin_ctzll(m);
+            __builtin_amdgcn_raw_buffer_store_b8(
+                (unsigned char)1, zr, (int)(z_lane + (unsigned)(t * kColStride)), 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b64(
+                (u32x2)(0u), pr, (int)(p_lane + (unsigned)(t * kColStride) * 8u), 0, 0);
+        }
+        if (live) {
+            int ta = __builtin_ctzll(live);
+            live &= live - 1ull;
+            if (ta != 0) load_batch(ta, xa, ca, ha);  // step 0's batch went out before the prologue
+            while (true) {
+                if (J.prio) set_issue_priority(__builtin_popcountll(live) + 1, nstep);
+                const int tb = live ? __builtin_ctzll(live) : nstep;
+                live &= live - 1ull;
+                if (tb < nstep) load_batch(tb, xb, cb, hb);
+                run_batch(ta, xa, ca, ha);
+                if (tb >= nstep) break;
+                if (J.prio) set_issue_priority(__builtin_popcountll(live) + 1, nstep);
+                ta = live ? __builtin_ctzll(live) : nstep;
+                live &= live - 1ull;
+                if (ta < nstep) load_batch(ta, xa, ca, ha);
+                run_batch(tb, xb, cb, hb);
+                if (ta >= nstep) break;
+            }
+        }
+    } else {
+        for (int t0 = 0; t0 < nstep; t0 += 2 * U) {
+            if (J.prio) set_issue_priority(nstep - t0, nstep);
+            if (t0 + U < nstep) load_batch(t0 + U, xb, cb, hb);
+            run_batch(t0, xa, ca, ha);
+            if (t0 + U >= nstep) break;
+            if (J.prio) set_issue_priority(nstep - t0 - U, nstep);
+            if (t0 + 2 * U < nstep) load_batch(t0 + 2 * U, xa, ca, ha);
+            run_batch(t0 + U, xb, cb, hb);
+        }
     }
     tile_partials_out<VEC, LPC, EVEN>(J, racc, oacc, s_row);
 }

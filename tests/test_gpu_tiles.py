@@ -72,7 +72,7 @@ def test_tile_trajectory_matches_oracle(device, tile, n_rows):
 
 
 @pytest.mark.parametrize("tile", [(4, 64, 2, 2, 2), (1, 32, 5, 0, 4), (4, 32, 8, 0, 2),
-                                  (8, 8, 6, 2, 1), (8, 16, 3, 0, 1)],
+                                  (8, 8, 6, 2, 1), (8, 16, 3, 0, 1), (8, 16, 16, 2, 1)],
                          ids=lambda t: "v%d_l%d_n%d_st%d_u%d" % t)
 def test_tile_full_solve_matches_oracle(device, tile):
     from reg_drot import _abi
