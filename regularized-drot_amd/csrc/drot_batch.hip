@@ -143,9 +143,11 @@ __global__ __launch_bounds__(kBlock) void batch_pass_kernel(QrBatchArgs A) {
     J.f1_out = SC ? A.f1pb + (size_t)(A.k & 1) * A.row_len + P.row_off : nullptr;
     const IterCtl ctl = batch_ctl(A, P, prob);
     // plain X stores: a batch's footprint lives in HBM, and write-through stores
-    // made the mixed 256-problem forward 5% slower (0.99 vs 0.94 s)
-    qr_tile<VEC, LPC, EVEN, 0, (VEC == 8 ? 1 : 2), ZS, SC>(J, ctl, bt.rt == 0 && bt.ct == 0, s_row,
-                                                       s_red);
+    // made the mixed 256-problem forward 5% slower (0.99 vs 0.94 s).  One
+    // column step per load batch: with screening the loop then visits only the
+    // steps a wave does not screen whole (mixed forward 722 vs 732 ms against
+    // two steps per batch; 64 x 512^2 unchanged)
+    qr_tile<VEC, LPC, EVEN, 0, 1, ZS, SC>(J, ctl, bt.rt == 0 && bt.ct == 0, s_row, s_red);
 }
 
 __global__ __launch_bounds__(kBlock) void batch_finalize_kernel(QrBatchArgs A) {

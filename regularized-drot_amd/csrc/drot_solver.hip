@@ -184,13 +184,17 @@ int gl_segments(const std::vector<int> &hb, int chunk, std::vector<int> *gseg,
 // to 1e-4 with / without them on MI355X: 1000^2 to 2048^2 1.01-1.03x,
 // 2500^2 0.99x, 3000^2 0.94x, 10000^2 0.69x per iteration.
 constexpr long long kZeroSkipMinElems = 6000000;
-// Screening of flagged segments (with the flags) from this many elements up,
-// for single problems and for a batch's total: it pays once C and X stream
-// from HBM rather than the 256 MB last-level cache.  MI355X, time to 1e-4 with
-// and without: 10000^2 269 vs 379 ms, 8000^2 250 vs 266, 5000^2 115 vs 106,
-// 3000^2 51 vs 45; mixed 256-problem batch forward 734 vs 814 ms, 64 x 512^2
-// forward 45 vs 42 ms.
-constexpr long long kScreenMinElems = 50000000;
+// Screening of flagged segments (with the flags) of single problems from this
+// many elements up, in tiles of one column step per load batch (their loop
+// visits only the steps a wave does not screen whole).  MI355X, time to 1e-4
+// with and without, two-chunk tiles: 10000^2 198 vs 378 ms, 8000^2 136 vs
+// 191, 6000^2 103 vs 133, 5000^2 76 vs 88, 4000^2 66.6 vs 65.8, 3000^2 50.5
+// vs 44.8 (C and X in the 256 MB last-level cache: the bound's bookkeeping
+// costs more than the C reads it saves).
+constexpr long long kScreenMinElems = 20000000;
+// The same for a batch's total elements: mixed 256-problem batch forward 734
+// vs 814 ms, 64 x 512^2 forward 45 vs 42 ms.
+constexpr long long kBatchScreenMinElems = 50000000;
 
 // Plan of a QR solve with a given pass tile; nstep <= 0 picks the column steps.
 Plan plan_qr_tile(int n_rows, int n_cols, QrTile t, int nstep) {
@@ -227,13 +231,15 @@ Plan plan_qr_tile(int n_rows, int n_cols, QrTile t, int nstep) {
     const bool zskip = nstep <= P.qt.lpc && want;
     P.zf_bytes = zskip ? (size_t)P.n_rt * (size_t)n_cols : 0;
     // screening of flagged segments, with the flags, in the branch-free tile
-    // (rows in whole 16-byte chunks for vec 4 and 8).  DROT_QR_SCREEN=1 / 0
-    // turns it on / off wherever the flags are on.
+    // (rows in whole 16-byte chunks for vec 4 and 8), by default in tiles of
+    // one step per batch.  DROT_QR_SCREEN=1 / 0 turns it on / off wherever
+    // the flags are on.
     const char *sc = getenv("DROT_QR_SCREEN");
     const bool lean = (P.qt.vec == 1 || n_rows % 4 == 0) &&
                       (long long)(P.tc + 64) * n_rows * 4 < (1LL << 31);
     const bool want_sc = (sc && sc[0] == '1') ||
-                         ((long long)n_rows * n_cols >= kScreenMinElems && !(sc && sc[0] == '0'));
+                         ((long long)n_rows * n_cols >= kScreenMinElems && P.qt.u == 1 &&
+                          !(sc && sc[0] == '0'));
     P.scr = (zskip && lean && want_sc) ? 1 : 0;
     layout(P);
     return P;
@@ -270,12 +276,14 @@ bool tuned_tile(int n_rows, int n_cols, bool v4, QrTile *t, int *nstep) {
 // tiles) and the fewest column steps (at most 8) whose grid fits one resident
 // round of the even pass (kResidentPass blocks: 5 per CU on 256 CUs), grown
 // while tiles hold under kSmallTile elements and the grid keeps kMinGrid
-// blocks.  A grid beyond two rounds even at 8 steps takes the tuned 10000^2
-// tile when the rows allow two 16-byte chunks per lane: 8 rows per lane, 16
-// lanes per column, 16 column steps, write-through stores while C and X fit
-// the 256 MB last-level cache and plain ones beyond (MI355X, per iteration
-// against the 32-lane tile below: 5000^2 28.2 vs 34.0 us); otherwise 32 lanes
-// per column, 8 steps and plain stores.
+// blocks.  A grid beyond one round even at 8 steps takes two 16-byte chunks
+// per lane when the rows allow it: 8 rows per lane, 16 lanes per column
+// (128-row tiles), 8 column steps while that grid fits one round, else 16
+// (the tuned 10000^2 tile), write-through stores while C and X fit the 256 MB
+// last-level cache and plain ones beyond.  MI355X, time to 1e-4: 4000^2 65.8
+// vs 76.2 ms (against 16 lanes, 8 steps, one chunk), 5000^2 88.1 vs 106.5
+// (against the 32-lane tile below), 8000^2 188.9 vs 250.0.  Other row counts
+// beyond two rounds: 32 lanes per column, 8 steps and plain stores.
 constexpr long long kResidentPass = 1280;
 constexpr long long kMinGrid = 512;
 constexpr int kSmallTile = 4096;
@@ -286,15 +294,16 @@ void builtin_tile16(int n_rows, int n_cols, QrTile *t, int *nstep) {
     auto grid = [&](int ns) { return n_rt * ceil_div(n_cols, ns * tc_min); };
     int ns = 1;
     while (ns < 8 && grid(ns) > kResidentPass) ns++;
+    const long long cx_bytes = 8LL * n_rows * n_cols;  // C and X
+    if (grid(ns) > kResidentPass && n_rows % 4 == 0 && (256LL + 64) * n_rows * 4 < (1LL << 31)) {
+        const long long grid8 = ceil_div(n_rows, 128) * (long long)ceil_div(n_cols, 128);
+        *t = QrTile{8, 16, cx_bytes < (200LL << 20) ? 2 : 0, 1};
+        *nstep = grid8 <= kResidentPass ? 8 : 16;
+        return;
+    }
     if (grid(ns) > 2 * kResidentPass) {
-        const long long cx_bytes = 8LL * n_rows * n_cols;  // C and X
-        if (n_rows % 4 == 0 && (256LL + 64) * n_rows * 4 < (1LL << 31)) {
-            *t = QrTile{8, 16, cx_bytes < (200LL << 20) ? 2 : 0, 1};
-            *nstep = 16;
-        } else {
-            *t = QrTile{4, 32, 0, 2};
-            *nstep = 8;
-        }
+        *t = QrTile{4, 32, 0, 2};
+        *nstep = 8;
         return;
     }
     while (ns < 8 && tr * tc_min * ns < kSmallTile && grid(ns + 1) >= kMinGrid) ns++;
@@ -1430,11 +1439,12 @@ drot_status plan_batch(int n, const drot_batch_problem *in, BatchPlan *out) {
     B.zf_len = (zs && zs[0] == '0') ? 0 : zf_off;
     B.off_zf = take(2 * (size_t)B.zf_len);
     // screening with the flags, for batches whose matrices add up to
-    // kScreenMinElems elements or more; DROT_QR_SCREEN=1 / 0 turns it on / off
+    // kBatchScreenMinElems elements or more; DROT_QR_SCREEN=1 / 0 turns it on / off
     const char *sc = getenv("DROT_QR_SCREEN");
     long long elems = 0;
     for (const auto &P : B.probs) elems += (long long)P.n_rows * P.n_cols;
-    const bool want_sc = (sc && sc[0] == '1') || (elems >= kScreenMinElems && !(sc && sc[0] == '0'));
+    const bool want_sc =
+        (sc && sc[0] == '1') || (elems >= kBatchScreenMinElems && !(sc && sc[0] == '0'));
     B.scr = (B.zf_len > 0 && want_sc) ? 1 : 0;
     B.off_seg_e = take(B.scr ? 4 * (size_t)B.zf_len : 0);
     B.off_g1 = take(B.scr ? 8 * (size_t)tile_off : 0);

@@ -227,10 +227,13 @@ def test_zero_segment_flags_in_batch_workspace(monkeypatch):
 
 
 def test_screening_buffers_in_qr_workspace(monkeypatch):
-    """QR solves from 50M elements (with the zero-segment flags, in the
-    branch-free tile) also reserve the screening state: an fp32 headroom per
-    flag, an fp64 bound per tile and two fp32 row buffers.  DROT_QR_SCREEN=1 /
-    0 turns it on / off wherever the flags are on."""
+    """QR solves that screen flagged segments also reserve the screening state:
+    an fp32 headroom per flag, an fp64 bound per tile and two fp32 row
+    buffers.  By default only tiles of one column step per load batch screen,
+    from 20M elements; DROT_QR_SCREEN=1 / 0 turns it on / off wherever the
+    flags are on.  The size covers the plans with and without 16-byte row
+    chunks, and the one-float-per-lane plan (two steps per batch, more row
+    tiles) is the larger."""
     from reg_drot import _abi
 
     def ws(n, sc):
@@ -244,8 +247,6 @@ def test_screening_buffers_in_qr_workspace(monkeypatch):
         return -(-b // 256) * 256
 
     def screen_bytes(n):
-        # the size covers the plans with and without 16-byte row chunks; the
-        # one-float-per-lane plan (more row tiles) is the larger
         t = _abi.qr_tile_for(n, n, False)
         tr = t.vec * t.lanes_per_col
         tc = t.col_steps * 4 * (64 // t.lanes_per_col)
@@ -253,12 +254,12 @@ def test_screening_buffers_in_qr_workspace(monkeypatch):
         return a256(n_rt * n * 4) + a256(n_rt * n_ct * 8) + a256(2 * n * 4)
 
     monkeypatch.delenv("DROT_QR_ZSKIP", raising=False)
-    n = 7072  # 50.01M elements
-    assert ws(n, None) - ws(n, "0") == screen_bytes(n)
-    assert ws(n, "1") == ws(n, None)
-    for n in (3000, 7000):  # below the threshold
-        assert ws(n, None) == ws(n, "0")
+    for n in (3000, 7072):  # flags on (from 6M elements)
+        assert _abi.qr_tile_for(n, n, False).unroll > 1
         assert ws(n, "1") - ws(n, "0") == screen_bytes(n)
+        # by default the larger plan, two steps per batch, does not screen
+        assert ws(n, None) == ws(n, "0")
+    assert _abi.qr_tile_for(7072, 7072).unroll == 1
     # no flags, no screening
     monkeypatch.setenv("DROT_QR_ZSKIP", "0")
     assert ws(7072, "1") == ws(7072, "0")
