@@ -116,7 +116,8 @@ struct QrTileJob {
     int prio;       // 1: waves raise their issue priority while many column steps remain
     float step, kappa;
     // zero-segment flags, [n_rt][n_cols] bytes, both null or both set
-    // (needs nstep <= LPC: lane rl of a column group loads the flag of step rl)
+    // (needs nstep <= LPC, 2 LPC in two-chunk tiles: lane rl of a column group
+    // loads the flag of step rl, and of step rl + LPC)
     const unsigned char *zin;
     unsigned char *zout;
     // screening (SC): the problem's [n_rt][n_cols] headrooms E, this tile's G1,
@@ -381,22 +382,36 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
     const int col_lim = tile_cols - jt0;  // step t is a column of the problem iff t * kColStride < col_lim
 
     // zero flags of this column group's segments from the previous pass, bit t
-    // for column step t (see qr_tile_generic)
-    using ZMask = std::conditional_t<(LPC > 32), unsigned long long, unsigned>;
+    // for column step t (see qr_tile_generic).  Lane rl of the column group
+    // loads the flag of step rl and, in two-chunk tiles (up to 2 LPC steps),
+    // of step rl + LPC as well (slot q: step rl + q LPC).
+    constexpr int kZSlots = VEC == 8 ? 2 : 1;
+    using ZMask = std::conditional_t<(kZSlots * LPC > 32), unsigned long long, unsigned>;
     constexpr int kZBits = 8 * (int)sizeof(ZMask);
     ZMask zmask = 0;
-    bool zb = false;
-    // screening data: lane rl holds step rl's headroom and column potential
-    float e_rel = INFINITY, h_scr = 0.0f;
+    bool zb[kZSlots];
+    // screening data: slot q of lane rl holds step rl + q LPC's headroom and
+    // column potential
+    float e_rel[kZSlots], h_scr[kZSlots];
+#pragma unroll
+    for (int q = 0; q < kZSlots; q++) {
+        zb[q] = false;
+        e_rel[q] = INFINITY;
+        h_scr[q] = 0.0f;
+    }
     if (ZS) {
-        const int jz = tile_col0 + jt0 + rl * kColStride;
-        const bool zok = rl < nstep && jz < M;
-        zb = zok && J.zin[(size_t)J.rt * M + jz] != 0;
-        if constexpr (SC) {
-            e_rel = zok ? J.seg_e[(size_t)J.rt * M + jz] : INFINITY;
-            h_scr = zok ? hat2[jz] : 0.0f;
+#pragma unroll
+        for (int q = 0; q < kZSlots; q++) {
+            const int ts = rl + q * LPC;
+            const int jz = tile_col0 + jt0 + ts * kColStride;
+            const bool zok = ts < nstep && jz < M;
+            zb[q] = zok && J.zin[(size_t)J.rt * M + jz] != 0;
+            if constexpr (SC) {
+                e_rel[q] = zok ? J.seg_e[(size_t)J.rt * M + jz] : INFINITY;
+                h_scr[q] = zok ? hat2[jz] : 0.0f;
+            }
+            zmask |= (ZMask)group_bits<LPC>(__builtin_amdgcn_ballot_w64(zb[q]), cs) << (q * LPC);
         }
-        zmask = (ZMask)group_bits<LPC>(__builtin_amdgcn_ballot_w64(zb), cs);
     }
     static_assert(!SC || ZS, "screening needs the zero-segment flags");
     double g1_old = 0.0;
@@ -503,17 +518,22 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
             for (int e = 0; e < VEC; e++)
                 if (rows[e] < N) J.f1_out[rows[e]] = f1[e];
         }
-        // f2 of step rl within an ulp or two of the pass's own (the bound's
-        // slack covers it); an unset headroom (+inf) never screens
-        const float f2s = explicit_phi ? h_scr : (h_scr + gamma) / fN;
-        const double bnd = (double)e_rel + g1_new + (double)f2s;
-        const double slack = 0x1p-21 * ((double)f1abs + fabs((double)f2s)) +
-                             0x1p-40 * (fabs((double)e_rel) + fabs(g1_new)) + 0x1p-120;
-        // a step past the problem's columns counts as screened as well: its
-        // loads read nothing and its stores are dropped either way
-        const bool past = rl < nstep && tile_col0 + jt0 + rl * kColStride >= M;
-        const bool sk = (zb && ctl.k > 0 && bnd + slack < 0.0) || past;
-        smask = (ZMask)group_bits<LPC>(__builtin_amdgcn_ballot_w64(sk), cs);
+#pragma unroll
+        for (int q = 0; q < kZSlots; q++) {
+            // f2 of the slot's step within an ulp or two of the pass's own
+            // (the bound's slack covers it); an unset headroom (+inf) never
+            // screens
+            const float f2s = explicit_phi ? h_scr[q] : (h_scr[q] + gamma) / fN;
+            const double bnd = (double)e_rel[q] + g1_new + (double)f2s;
+            const double slack = 0x1p-21 * ((double)f1abs + fabs((double)f2s)) +
+                                 0x1p-40 * (fabs((double)e_rel[q]) + fabs(g1_new)) + 0x1p-120;
+            // a step past the problem's columns counts as screened as well:
+            // its loads read nothing and its stores are dropped either way
+            const int ts = rl + q * LPC;
+            const bool past = ts < nstep && tile_col0 + jt0 + ts * kColStride >= M;
+            const bool sk = (zb[q] && ctl.k > 0 && bnd + slack < 0.0) || past;
+            smask |= (ZMask)group_bits<LPC>(__builtin_amdgcn_ballot_w64(sk), cs) << (q * LPC);
+        }
     }
     // headroom of a segment whose C this pass reads, stored by lane 0 of the
     // column group: max_i RN(f1_i - s c_ij) = R bounds the exact maximum by

@@ -223,12 +223,13 @@ Plan plan_qr_tile(int n_rows, int n_cols, QrTile t, int nstep) {
     // zero-segment flags (drot_qr_tile.h) from kZeroSkipMinElems elements up:
     // they save HBM traffic, while smaller matrices are bound by latency,
     // which the flag load at the start of a tile adds to.  A column group
-    // gathers its steps' flags one per lane, so nstep <= lpc.
-    // DROT_QR_ZSKIP=1 / 0 turns them on / off for every size.
+    // gathers its steps' flags one per lane (two in two-chunk tiles), so
+    // nstep <= lpc (2 lpc).  DROT_QR_ZSKIP=1 / 0 turns them on / off for
+    // every size.
     const char *zs = getenv("DROT_QR_ZSKIP");
     const bool big = (long long)n_rows * n_cols >= kZeroSkipMinElems;
     const bool want = (zs && zs[0] == '1') || (big && !(zs && zs[0] == '0'));
-    const bool zskip = nstep <= P.qt.lpc && want;
+    const bool zskip = nstep <= (P.qt.vec == 8 ? 2 : 1) * P.qt.lpc && want;
     P.zf_bytes = zskip ? (size_t)P.n_rt * (size_t)n_cols : 0;
     // screening of flagged segments, with the flags, in the branch-free tile
     // (rows in whole 16-byte chunks for vec 4 and 8), by default in tiles of
@@ -278,13 +279,16 @@ bool tuned_tile(int n_rows, int n_cols, bool v4, QrTile *t, int *nstep) {
 // while tiles hold under kSmallTile elements and the grid keeps kMinGrid
 // blocks.  A grid beyond one round even at 8 steps takes two 16-byte chunks
 // per lane when the rows allow it: 8 rows per lane, 16 lanes per column
-// (128-row tiles), 8 column steps while that grid fits one round, else 16
-// (the tuned 10000^2 tile), write-through stores while C and X fit the 256 MB
-// last-level cache and plain ones beyond.  MI355X, time to 1e-4: 4000^2 65.8
-// vs 76.2 ms (against 16 lanes, 8 steps, one chunk), 5000^2 88.1 vs 106.5
-// (against the 32-lane tile below), 8000^2 188.9 vs 250.0.  Other row counts
-// beyond two rounds: 32 lanes per column, 8 steps and plain stores.
+// (128-row tiles) and the fewest of 8, 16, 24 or 32 column steps whose grid
+// fits one round of these tiles (kResidentTwoChunk blocks: four per CU), 32
+// when none does; write-through stores while C and X fit the 256 MB
+// last-level cache and plain ones beyond.  MI355X, time to 1e-4: 4000^2 65.7
+// vs 76.2 ms (against 16 lanes, 8 steps, one chunk), 5000^2 75.9 (16 steps;
+// 84.0 at 32), 6000^2 86.4 (24 steps; 106.5 at 16, 95.8 at 32), 8000^2 115.9
+// (32 steps; 137.1 at 16), 10000^2 166.6 (32 steps; 200.2 at 16).  Other row
+// counts beyond two rounds: 32 lanes per column, 8 steps and plain stores.
 constexpr long long kResidentPass = 1280;
+constexpr long long kResidentTwoChunk = 1024;
 constexpr long long kMinGrid = 512;
 constexpr int kSmallTile = 4096;
 
@@ -295,10 +299,13 @@ void builtin_tile16(int n_rows, int n_cols, QrTile *t, int *nstep) {
     int ns = 1;
     while (ns < 8 && grid(ns) > kResidentPass) ns++;
     const long long cx_bytes = 8LL * n_rows * n_cols;  // C and X
-    if (grid(ns) > kResidentPass && n_rows % 4 == 0 && (256LL + 64) * n_rows * 4 < (1LL << 31)) {
-        const long long grid8 = ceil_div(n_rows, 128) * (long long)ceil_div(n_cols, 128);
+    if (grid(ns) > kResidentPass && n_rows % 4 == 0 && (512LL + 64) * n_rows * 4 < (1LL << 31)) {
+        int steps = 8;
+        while (steps < 32 &&
+               ceil_div(n_rows, 128) * (long long)ceil_div(n_cols, 16 * steps) > kResidentTwoChunk)
+            steps += 8;
         *t = QrTile{8, 16, cx_bytes < (200LL << 20) ? 2 : 0, 1};
-        *nstep = grid8 <= kResidentPass ? 8 : 16;
+        *nstep = steps;
         return;
     }
     if (grid(ns) > 2 * kResidentPass) {
@@ -333,7 +340,7 @@ std::vector<std::pair<QrTile, int>> tune_candidates(int n_rows, int n_cols, bool
     // two-chunk tiles: 8 or 16 lanes per column, one step per batch
     for (int lpc : {8, 16})
         for (int nt : {2, 0})
-            for (int nstep : {1, 2, 3, 4, 5, 6, 7, 8, 12, 16}) {
+            for (int nstep : {1, 2, 3, 4, 5, 6, 7, 8, 12, 16, 24, 32}) {
                 const QrTile t{8, lpc, nt, 1};
                 const int tc = nstep * kWavesPerBlock * (kWave / lpc);
                 if (lpc * 8 > 4 * std::max(n_rows, 16) || tc > 4 * std::max(n_cols, 4)) continue;

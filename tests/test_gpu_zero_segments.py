@@ -119,10 +119,12 @@ def test_flags_keep_bits_every_tile(device, tile):
         _flags_keep_bits(c, p, q, prm, device, tile)
 
 
-@pytest.mark.parametrize("tile", ["8,8,8,2,1", "8,16,5,0,1", "8,8,3,0,1", "8,16,16,2,1"])
+@pytest.mark.parametrize("tile", ["8,8,8,2,1", "8,16,5,0,1", "8,8,3,0,1", "8,16,16,2,1",
+                                  "8,8,13,2,1", "8,16,24,2,1", "8,16,32,0,1"])
 def test_flags_keep_bits_two_chunk_tiles(device, tile, monkeypatch):
     """Two 4-row chunks per lane (rows a multiple of 4; the last row tile is
-    partial, so one chunk of a lane can lie inside and the other past n_rows)."""
+    partial, so one chunk of a lane can lie inside and the other past n_rows).
+    Beyond LPC column steps a lane holds the flags of two steps."""
     from reg_drot import _abi
     n_rows, n_cols = 1004, 641
     monkeypatch.setenv("DROT_QR_TILE", tile)
