@@ -104,7 +104,6 @@ __global__ __launch_bounds__(kBlock) void batch_init_kernel(QrBatchArgs A) {
 template <bool EVEN, bool ZS, bool SC, int VEC, int LPC>
 __global__ __launch_bounds__(kBlock) void batch_pass_kernel(QrBatchArgs A) {
     constexpr int CPW = kWave / LPC;
-    constexpr int kSteps = kBatchTileCols / (kWavesPerBlock * CPW);
     static_assert(VEC * LPC == kBatchTileRows, "64-row batch tiles");
     __shared__ double s_row[kWavesPerBlock * CPW][kBatchTileRows];
     __shared__ double s_red[3 * kWavesPerBlock];
@@ -130,9 +129,9 @@ __global__ __launch_bounds__(kBlock) void batch_pass_kernel(QrBatchArgs A) {
     J.ld = P.ld;
     J.rt = bt.rt;
     J.ct = bt.ct;
-    J.nstep = kSteps;
+    J.nstep = P.nstep;
     J.prio = 0;
-    J.tc = kBatchTileCols;
+    J.tc = P.nstep * kWavesPerBlock * CPW;
     J.step = P.step;
     J.kappa = P.kappa;
     J.zin = ZS ? A.zf + (size_t)((A.k + 1) & 1) * A.zf_len + P.zf_off : nullptr;

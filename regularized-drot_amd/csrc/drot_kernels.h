@@ -199,7 +199,8 @@ struct QrBatchArgs {
 constexpr int kBatchVec = 4;
 constexpr int kBatchLpc = 16;
 constexpr int kBatchTileRows = kBatchLpc * kBatchVec;
-constexpr int kBatchTileCols = 256;
+constexpr int kBatchTileCols = 256;          // 512 in batches of 50M elements and more
+constexpr int kBatchTileColsScreened = 512;
 
 int batch_pass_grid(int n_tiles);
 hipError_t launch_batch_init(const QrBatchArgs &a, hipStream_t s);

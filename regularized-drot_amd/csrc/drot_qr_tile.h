@@ -116,8 +116,8 @@ struct QrTileJob {
     int prio;       // 1: waves raise their issue priority while many column steps remain
     float step, kappa;
     // zero-segment flags, [n_rt][n_cols] bytes, both null or both set
-    // (needs nstep <= LPC, 2 LPC in two-chunk tiles: lane rl of a column group
-    // loads the flag of step rl, and of step rl + LPC)
+    // (needs nstep <= LPC, 2 LPC in the branch-free 16-byte tiles: lane rl of
+    // a column group loads the flag of step rl, and of step rl + LPC)
     const unsigned char *zin;
     unsigned char *zout;
     // screening (SC): the problem's [n_rt][n_cols] headrooms E, this tile's G1,
@@ -383,9 +383,9 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
 
     // zero flags of this column group's segments from the previous pass, bit t
     // for column step t (see qr_tile_generic).  Lane rl of the column group
-    // loads the flag of step rl and, in two-chunk tiles (up to 2 LPC steps),
-    // of step rl + LPC as well (slot q: step rl + q LPC).
-    constexpr int kZSlots = VEC == 8 ? 2 : 1;
+    // loads the flag of step rl and, in 16-byte tiles (up to 2 LPC steps), of
+    // step rl + LPC as well (slot q: step rl + q LPC).
+    constexpr int kZSlots = VEC >= 4 ? 2 : 1;
     using ZMask = std::conditional_t<(kZSlots * LPC > 32), unsigned long long, unsigned>;
     constexpr int kZBits = 8 * (int)sizeof(ZMask);
     ZMask zmask = 0;

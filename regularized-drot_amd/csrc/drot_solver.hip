@@ -1353,6 +1353,36 @@ drot_status plan_batch(int n, const drot_batch_problem *in, BatchPlan *out) {
     constexpr int tc_min = kWavesPerBlock * cpw;
     int tile_off = 0, fin_off = 0;
     long long prow_off = 0, pcol_off = 0, zf_off = 0;
+    // zero-segment flags for every problem (the batch keeps the HBM busy, so
+    // skipping settled segments pays at all sizes); DROT_QR_ZSKIP=0 turns them
+    // off.  Screening with the flags for batches whose matrices add up to
+    // kBatchScreenMinElems elements or more; DROT_QR_SCREEN=1 / 0 turns it on
+    // / off.
+    const char *zs = getenv("DROT_QR_ZSKIP");
+    const bool flags = !(zs && zs[0] == '0');
+    const char *sc = getenv("DROT_QR_SCREEN");
+    long long elems = 0;
+    for (int i = 0; i < n; i++)
+        elems += (long long)std::max(in[i].n_rows, 0) * (long long)std::max(in[i].n_cols, 0);
+    const bool want_sc =
+        (sc && sc[0] == '1') || (elems >= kBatchScreenMinElems && !(sc && sc[0] == '0'));
+    B.scr = (flags && want_sc) ? 1 : 0;
+    // 256-column tiles: with zero-segment flags a block's cost is mostly its
+    // latency chain (flag load, first data, prologue), which longer tiles
+    // spread over more columns (mixed 256-problem forward with 4-row lane
+    // slices: 1.21 s at 64 columns, 1.04 s at 128, 0.99 s at 256).  With
+    // screening most steps of a settled tile are skipped whole, and 512
+    // columns (two flags per lane) spread the chain further: mixed forward
+    // 685 vs 717 ms; the unscreened 64 x 512^2 forward is faster at 256
+    // (41.9 vs 48.4 ms: more blocks).  The width follows the batch's size
+    // rather than the screening switch, so that a batch keeps its tiles, and
+    // its bits, with screening on or off; DROT_BATCH_TILE_COLS=256 / 512
+    // sets it.
+    const char *bw = getenv("DROT_BATCH_TILE_COLS");
+    const int tc = bw && atoi(bw) == kBatchTileColsScreened ? kBatchTileColsScreened
+                   : bw && atoi(bw) == kBatchTileCols       ? kBatchTileCols
+                   : elems >= kBatchScreenMinElems          ? kBatchTileColsScreened
+                                                            : kBatchTileCols;
     for (int i = 0; i < n; i++) {
         const drot_batch_problem &d = in[i];
         if (d.n_rows <= 0 || d.n_cols <= 0)
@@ -1372,14 +1402,9 @@ drot_status plan_batch(int n, const drot_batch_problem *in, BatchPlan *out) {
         P.n_cols = d.n_cols;
         P.ld = d.ld;
         P.n_rt = ceil_div(d.n_rows, kBatchTileRows);
-        // 256-column tiles: with zero-segment flags a block's cost is mostly
-        // its latency chain (flag load, first data, prologue), which longer
-        // tiles spread over more columns (mixed 256-problem forward with 4-row
-        // lane slices: 1.21 s at 64 columns, 1.04 s at 128, 0.99 s at 256).
         // The tile's column range plus the load batches' slack of 64 columns
         // must stay under 2^31 bytes (the branch-free tile's buffer offsets).
-        P.nstep = kBatchTileCols / tc_min;
-        const int tc = kBatchTileCols;
+        P.nstep = tc / tc_min;
         if ((long long)(tc + 64) * d.ld * 4 >= (1LL << 31))
             return fail(DROT_ERR_UNSUPPORTED, "problem " + std::to_string(i) + ": ld too large");
         P.n_ct = ceil_div(d.n_cols, tc);
@@ -1440,19 +1465,8 @@ drot_status plan_batch(int n, const drot_batch_problem *in, BatchPlan *out) {
     B.off_probs = take(sizeof(BatchProblem) * (size_t)n);
     B.off_tiles = take(sizeof(BatchTile) * B.tiles.size());
     B.off_fins = take(sizeof(BatchFin) * B.fins.size());
-    // zero-segment flags for every problem (the batch keeps the HBM busy, so
-    // skipping settled segments pays at all sizes); DROT_QR_ZSKIP=0 turns them off
-    const char *zs = getenv("DROT_QR_ZSKIP");
-    B.zf_len = (zs && zs[0] == '0') ? 0 : zf_off;
+    B.zf_len = flags ? zf_off : 0;
     B.off_zf = take(2 * (size_t)B.zf_len);
-    // screening with the flags, for batches whose matrices add up to
-    // kBatchScreenMinElems elements or more; DROT_QR_SCREEN=1 / 0 turns it on / off
-    const char *sc = getenv("DROT_QR_SCREEN");
-    long long elems = 0;
-    for (const auto &P : B.probs) elems += (long long)P.n_rows * P.n_cols;
-    const bool want_sc =
-        (sc && sc[0] == '1') || (elems >= kBatchScreenMinElems && !(sc && sc[0] == '0'));
-    B.scr = (B.zf_len > 0 && want_sc) ? 1 : 0;
     B.off_seg_e = take(B.scr ? 4 * (size_t)B.zf_len : 0);
     B.off_g1 = take(B.scr ? 8 * (size_t)tile_off : 0);
     B.off_f1p = take(B.scr ? 2 * 4 * (size_t)B.row_len : 0);

@@ -138,10 +138,13 @@ def test_flags_keep_bits_two_chunk_tiles(device, tile, monkeypatch):
         _flags_keep_bits(c, p, q, prm, device, tile)
 
 
+@pytest.mark.parametrize("tile_cols", ["256", "512"])
 @pytest.mark.parametrize("max_iters,eps", [(20000, 1e-4), (151, 0.0)])
-def test_batch_flags_keep_bits(device, max_iters, eps):
+def test_batch_flags_keep_bits(device, max_iters, eps, tile_cols, monkeypatch):
     """The batched solver flags every problem's segments (offsets into one
-    flag array, parity from the batch's iteration): same bits as without."""
+    flag array, parity from the batch's iteration): same bits as without.
+    Both tile widths: 512 columns (large batches) give a lane the flags of
+    two column steps."""
     import torch
     from reg_drot import batch
     shapes = [(64, 64), (130, 7), (257, 129), (1000, 800), (96, 2048), (2048, 96), (513, 511)]
@@ -152,6 +155,7 @@ def test_batch_flags_keep_bits(device, max_iters, eps):
         p, q = problems.uniform_marginals(n, m)
         ps.append(torch.from_numpy(p).to(device))
         qs.append(torch.from_numpy(q).to(device))
+    monkeypatch.setenv("DROT_BATCH_TILE_COLS", tile_cols)
     out = {}
     old = {k: os.environ.get(k) for k in ("DROT_QR_ZSKIP", "DROT_QR_SCREEN")}
     try:
@@ -159,7 +163,7 @@ def test_batch_flags_keep_bits(device, max_iters, eps):
             os.environ["DROT_QR_ZSKIP"] = zs
             os.environ["DROT_QR_SCREEN"] = sc
             print(f"[zero segments] batch of {len(shapes)} zskip={zs} screen={sc} "
-                  f"max_iters={max_iters}", flush=True)
+                  f"max_iters={max_iters} tile_cols={tile_cols}", flush=True)
             plans, infos = batch.quadratic_drot_batch(cs, ps, qs, 2.0, rws, max_iters, eps)
             torch.cuda.synchronize(device)
             out[zs + sc] = ([x.cpu().numpy() for x in plans], infos)
