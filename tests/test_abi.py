@@ -109,6 +109,27 @@ def test_zero_segment_flags_in_qr_workspace(monkeypatch):
         assert ws(n, "1") - ws(n, "0") == flag_bytes(n)
 
 
+def test_builtin_tile_rule_for_large_grids():
+    """Shapes whose 64-row, 8-step grid exceeds one resident round take
+    two-chunk tiles (8 rows per lane, 16 lanes per column) with the fewest of
+    8, 16, 24 or 32 column steps whose grid fits 1024 blocks (32 when none
+    does), write-through stores while C and X stay under 200 MB; rows not a
+    multiple of 4 keep one chunk per lane."""
+    from reg_drot import _abi
+    _abi.qr_tile_cache_clear()
+    expect = {3500: (8, 16, 8, 2, 1), 4000: (8, 16, 8, 2, 1), 4500: (8, 16, 16, 2, 1),
+              5000: (8, 16, 16, 2, 1), 6000: (8, 16, 24, 0, 1), 7000: (8, 16, 32, 0, 1),
+              10000: (8, 16, 32, 0, 1)}
+    for n, tile in expect.items():
+        assert _abi.qr_tile_for(n, n).as_tuple() == tile, n
+    for n, ns in expect.items():  # the chosen grid fits one round unless 32 steps do not
+        t = _abi.qr_tile_for(n, n)
+        grid = -(-n // 128) * -(-n // (16 * t.col_steps))
+        assert grid <= 1024 or t.col_steps == 32
+    assert _abi.qr_tile_for(4001, 4001).vec == 4
+    assert _abi.qr_tile_for(3000, 3000).vec == 4  # one round at 8 steps: one chunk
+
+
 def test_argument_errors_are_status_codes():
     from reg_drot import _abi
     L = lib()
