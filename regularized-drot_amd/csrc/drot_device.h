@@ -278,8 +278,42 @@ struct IterStart {
     float gamma;        // gamma of iteration k-1: phi = (hat + gamma) / n
 };
 
+// The first 4 kWave finalize entries of the prologue below, loaded by wave 0
+// ahead of time: lane l holds entries l, l + 64, l + 128, l + 192 (0 past
+// n_fin, and in the other waves).  Buffer loads with out-of-range offsets for
+// the lanes with nothing to read, so that no lane condition turns them into a
+// branch that waits.  Issued before a tile's first batch of X and C, the
+// prologue's wait for them does not also wait for that batch (a wave's loads
+// complete in order).
+struct PrologueLoads {
+    double v[4][3];
+    float alpha_prev;  // SolverState::alpha_slot[k & 1]
+};
+
+__device__ __forceinline__ void prologue_issue(const IterCtl &C, PrologueLoads &L) {
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<double *>(C.pfin), 0, C.k > 0 ? C.n_fin * 24 : 0, 0x00020000);
+    const bool w0 = threadIdx.x < kWave;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int t = (int)threadIdx.x + u * kWave;
+        const unsigned off = (w0 && t < C.n_fin) ? (unsigned)t * 24u : 0x80000000u;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            typedef unsigned int u2_t __attribute__((ext_vector_type(2)));
+            const u2_t d = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(off + 8u * c), 0, 0);
+            L.v[u][c] = __longlong_as_double((long long)(((unsigned long long)d.y << 32) | d.x));
+        }
+    }
+    const __amdgpu_buffer_rsrc_t ar =
+        __builtin_amdgcn_make_buffer_rsrc(C.st->alpha_slot, 0, 8, 0x00020000);
+    L.alpha_prev = __uint_as_float(
+        __builtin_amdgcn_raw_buffer_load_b32(ar, (int)((C.k & 1) * 4), 0, 0));
+}
+
 // Prologue of the pass of iteration k, called by every thread of every block;
-// s_red holds 3 * kWavesPerBlock doubles.
+// s_red holds 3 * kWavesPerBlock doubles.  pre: the first entries, issued
+// by prologue_issue (null: loaded here).
 // Finishes iteration k-1 from the per-block partials of its finalize launch,
 // identically in every block (same fixed-order sums):
 //   S = sum X', residual = sqrt(sum rs^2 + sum cs^2), objective,
@@ -289,7 +323,8 @@ struct IterStart {
 // core/drot_glr.hpp:344-354).  The problem's leader block records the result.  Because the test
 // runs before any block writes X, a converged iterate is never overwritten.
 __device__ inline IterStart iteration_prologue(const IterCtl &C, double *s_red,
-                                              bool leader_block) {
+                                              bool leader_block,
+                                              const PrologueLoads *pre = nullptr) {
     SolverState *st = C.st;
     IterStart r;
     r.explicit_phi = true;
@@ -307,17 +342,26 @@ __device__ inline IterStart iteration_prologue(const IterCtl &C, double *s_red,
         // (W = kWavesPerBlock) are ones that block_sum, the only later user of
         // s_red in a pass, never writes.
         if (threadIdx.x < kWave) {
-            const float alpha_prev = st->alpha_slot[k & 1];  // after iteration k-2
+            const float alpha_prev = pre ? pre->alpha_prev : st->alpha_slot[k & 1];  // after k-2
             double rr = 0.0, ss = 0.0, oo = 0.0;
             for (int t0 = (int)threadIdx.x; t0 < C.n_fin; t0 += 4 * kWave) {
                 double v[4][3];
+                if (pre && t0 == (int)threadIdx.x) {
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int t = t0 + u * kWave;
-                    const bool ok = t < C.n_fin;
-                    v[u][0] = ok ? C.pfin[3 * t] : 0.0;
-                    v[u][1] = ok ? C.pfin[3 * t + 1] : 0.0;
-                    v[u][2] = ok ? C.pfin[3 * t + 2] : 0.0;
+                    for (int u = 0; u < 4; u++) {
+                        v[u][0] = pre->v[u][0];
+                        v[u][1] = pre->v[u][1];
+                        v[u][2] = pre->v[u][2];
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int t = t0 + u * kWave;
+                        const bool ok = t < C.n_fin;
+                        v[u][0] = ok ? C.pfin[3 * t] : 0.0;
+                        v[u][1] = ok ? C.pfin[3 * t + 1] : 0.0;
+                        v[u][2] = ok ? C.pfin[3 * t + 2] : 0.0;
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < 4; u++) {

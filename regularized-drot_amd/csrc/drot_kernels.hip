@@ -24,6 +24,14 @@ struct VecIO;
 
 template <>
 struct VecIO<4> {
+    // raw buffer load: an offset past the resource's range reads 0
+    static __device__ __forceinline__ void load_buf(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                                    float (&v)[4]) {
+        typedef unsigned int u4_t __attribute__((ext_vector_type(4)));
+        const u4_t d = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+        v[0] = __uint_as_float(d.x); v[1] = __uint_as_float(d.y);
+        v[2] = __uint_as_float(d.z); v[3] = __uint_as_float(d.w);
+    }
     static __device__ __forceinline__ void load(const float *p, float (&v)[4]) {
         const float4 t = *reinterpret_cast<const float4 *>(p);
         v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
@@ -35,9 +43,28 @@ struct VecIO<4> {
 
 template <>
 struct VecIO<1> {
+    static __device__ __forceinline__ void load_buf(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                                    float (&v)[1]) {
+        v[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+    }
     static __device__ __forceinline__ void load(const float *p, float (&v)[1]) { v[0] = *p; }
     static __device__ __forceinline__ void store(float *p, const float (&v)[1]) { *p = v[0]; }
 };
+
+// raw buffer load of VEC ints: an offset past the resource's range reads 0
+template <int VEC>
+__device__ __forceinline__ void load_ints_buf(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                              int (&v)[VEC]) {
+    if constexpr (VEC == 4) {
+        typedef unsigned int u4_t __attribute__((ext_vector_type(4)));
+        const u4_t d = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+        v[0] = (int)d.x; v[1] = (int)d.y; v[2] = (int)d.z; v[3] = (int)d.w;
+    } else {
+#pragma unroll
+        for (int e = 0; e < VEC; e++)
+            v[e] = (int)__builtin_amdgcn_raw_buffer_load_b32(r, (int)(off + 4u * e), 0, 0);
+    }
+}
 
 template <int VEC>
 __device__ __forceinline__ void load_ints(const int *p, int (&v)[VEC]) {
@@ -535,51 +562,75 @@ __global__ __launch_bounds__(kBlock) void gl_rows_kernel(GlPassArgs A) {
     const float *hat2 = explicit_phi ? A.phi2 : A.bhat;
     const int j0 = blockIdx.x * A.tc;
     const int j1 = min(j0 + A.tc, M);
-    if (threadIdx.x == 0) s_done = A.ctl.st->done;
+    // Every load before the prologue is a buffer load whose offset is pushed
+    // out of range for a lane with nothing to read (the hardware returns 0),
+    // in this order: the prologue's finalize entries, the group bounds, the
+    // rows' potentials and groups, the first round of columns.  A load under
+    // a lane condition becomes a branch that waits for it and for everything
+    // issued before it (a wave's loads complete in order), which serialised
+    // the pass's start; in this order each wait covers only what it needs.
+    PrologueLoads pre;
+    prologue_issue(A.ctl, pre);
+    const int done_v = A.ctl.st->done;  // uniform: a scalar load
+    const bool bnd_lds = G <= kGlBoundsLds;
+    const __amdgpu_buffer_rsrc_t br =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<int *>(A.bounds), 0, (G + 1) * 4, 0x00020000);
+    const int bnd_v = (int)__builtin_amdgcn_raw_buffer_load_b32(
+        br, (int)(bnd_lds && (int)threadIdx.x <= G ? threadIdx.x * 4u : 0x80000000u), 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    const __amdgpu_buffer_rsrc_t h1r =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(hsrc), 0, N * 4, 0x00020000);
+    const __amdgpu_buffer_rsrc_t gr =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<int *>(A.row_group), 0, N * 4, 0x00020000);
     float h1[RCH][VEC];
     int gid[RCH][VEC];
 #pragma unroll
     for (int r = 0; r < RCH; r++) {
         const int i0 = (r * kBlock + (int)threadIdx.x) * VEC;
-#pragma unroll
-        for (int e = 0; e < VEC; e++) { h1[r][e] = 0.0f; gid[r][e] = 0; }
-        if (i0 < N) {
-            VecIO<VEC>::load(hsrc + i0, h1[r]);
-            load_ints<VEC>(A.row_group + i0, gid[r]);
-        }
+        const unsigned off = i0 < N ? (unsigned)i0 * 4u : 0x80000000u;
+        VecIO<VEC>::load_buf(h1r, off, h1[r]);
+        load_ints_buf<VEC>(gr, off, gid[r]);
     }
     // one register set: a second one, to keep the next round's loads in
     // flight, took the register count from 168 to 246 and made the pass
     // slower at every shape measured
     float xv[kCols][RCH][VEC], cv[kCols][RCH][VEC], h2[kCols];
+    // the block's columns [j0, j1) of X and C, and of the column potentials
+    const unsigned blk_bytes = (unsigned)(j1 - j0) * (unsigned)N * 4u;
+    const __amdgpu_buffer_rsrc_t xr =
+        __builtin_amdgcn_make_buffer_rsrc(A.x + (size_t)j0 * N, 0, (int)blk_bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t cr = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float *>(A.c) + (size_t)j0 * N, 0, (int)blk_bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t h2r = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float *>(hat2) + j0, 0, (j1 - j0) * 4, 0x00020000);
     auto load_round = [&](int jr) __attribute__((always_inline)) {
 #pragma unroll
         for (int jj = 0; jj < kCols; jj++) {
-            const int j = jr + jj;
-            const bool col_ok = j < j1;
-            h2[jj] = col_ok ? hat2[j] : 0.0f;
-            const float *xc = A.x + (size_t)j * N;
-            const float *cc = A.c + (size_t)j * N;
+            const int jc = jr + jj - j0;  // column within the block
+            const bool col_ok = jr + jj < j1;
+            h2[jj] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                h2r, (int)(col_ok ? (unsigned)jc * 4u : 0x80000000u), 0, 0));
 #pragma unroll
             for (int r = 0; r < RCH; r++) {
                 const int i0 = (r * kBlock + (int)threadIdx.x) * VEC;
-                if (col_ok && i0 < N) {
-                    VecIO<VEC>::load(xc + i0, xv[jj][r]);
-                    VecIO<VEC>::load(cc + i0, cv[jj][r]);
-                } else {
-#pragma unroll
-                    for (int e = 0; e < VEC; e++) { xv[jj][r][e] = 0.0f; cv[jj][r][e] = 0.0f; }
-                }
+                const unsigned off =
+                    (col_ok && i0 < N) ? ((unsigned)jc * (unsigned)N + (unsigned)i0) * 4u : 0x80000000u;
+                VecIO<VEC>::load_buf(xr, off, xv[jj][r]);
+                VecIO<VEC>::load_buf(cr, off, cv[jj][r]);
             }
         }
     };
     load_round(j0);
-    stage_bounds(A, s_bnd);  // its barrier also publishes s_done
-    if (s_done) return;      // the solve ended before this launch
-    const IterStart is = iteration_prologue(A.ctl, s_red, blockIdx.x == 0);
+    if (threadIdx.x == 0) s_done = done_v;
+    if (bnd_lds && (int)threadIdx.x <= G) s_bnd[threadIdx.x] = bnd_v;
+    if (bnd_lds && G >= kBlock) {  // more bounds than threads (rare): the rest directly
+        for (int g = threadIdx.x + kBlock; g <= G; g += kBlock) s_bnd[g] = A.bounds[g];
+    }
+    __syncthreads();
+    if (s_done) return;  // the solve ended before this launch
+    const IterStart is = iteration_prologue(A.ctl, s_red, blockIdx.x == 0, &pre);
     if (is.finish) return;
     const float gamma = is.gamma;
-    const bool bnd_lds = G <= kGlBoundsLds;
     auto bnd = [&](int g) { return bnd_lds ? s_bnd[g] : A.bounds[g]; };
     const float fM = (float)M, fN = (float)N;
     const float s = A.step;

@@ -381,6 +381,13 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
     const unsigned z_lane = rl == 0 ? (unsigned)jt0 : kOob;
     const int col_lim = tile_cols - jt0;  // step t is a column of the problem iff t * kColStride < col_lim
 
+    // the prologue's first loads go out first: waiting for the flags (needed
+    // to form the first batch) then brings them too, and the prologue's sums
+    // run while the first batch is in flight
+    PrologueLoads pre;
+    prologue_issue(ctl, pre);
+    __builtin_amdgcn_sched_barrier(0);
+
     // zero flags of this column group's segments from the previous pass, bit t
     // for column step t (see qr_tile_generic).  Lane rl of the column group
     // loads the flag of step rl and, in 16-byte tiles (up to 2 LPC steps), of
@@ -399,17 +406,34 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
         e_rel[q] = INFINITY;
         h_scr[q] = 0.0f;
     }
+    // The loads at the start of a tile are buffer loads whose offset is
+    // pushed out of range for a lane with nothing to read (the hardware
+    // returns 0): a load under a lane condition becomes a branch that waits
+    // for it, and for every load issued before it, on the spot, which
+    // serialised the tile's start behind its first batch.
     if (ZS) {
+        const __amdgpu_buffer_rsrc_t zir = buffer_over(J.zin + part_base, (unsigned)tile_cols);
+        const __amdgpu_buffer_rsrc_t eir =
+            SC ? buffer_over(J.seg_e + part_base, (unsigned)tile_cols * 4u) : zir;
+        bool zok[kZSlots];
+        unsigned char zv[kZSlots];
 #pragma unroll
         for (int q = 0; q < kZSlots; q++) {
             const int ts = rl + q * LPC;
-            const int jz = tile_col0 + jt0 + ts * kColStride;
-            const bool zok = ts < nstep && jz < M;
-            zb[q] = zok && J.zin[(size_t)J.rt * M + jz] != 0;
+            const int jc = jt0 + ts * kColStride;  // column within the tile
+            zok[q] = ts < nstep && jc < tile_cols;
+            const unsigned zo = zok[q] ? (unsigned)jc : kOob;
+            const unsigned zo4 = zok[q] ? (unsigned)jc * 4u : kOob;
+            zv[q] = __builtin_amdgcn_raw_buffer_load_b8(zir, (int)zo, 0, 0);
             if constexpr (SC) {
-                e_rel[q] = zok ? J.seg_e[(size_t)J.rt * M + jz] : INFINITY;
-                h_scr[q] = zok ? hat2[jz] : 0.0f;
+                e_rel[q] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(eir, (int)zo4, 0, 0));
+                h_scr[q] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(hr, (int)zo4, 0, 0));
             }
+        }
+#pragma unroll
+        for (int q = 0; q < kZSlots; q++) {
+            zb[q] = zok[q] && zv[q] != 0;
+            if constexpr (SC) e_rel[q] = zok[q] ? e_rel[q] : INFINITY;
             zmask |= (ZMask)group_bits<LPC>(__builtin_amdgcn_ballot_w64(zb[q]), cs) << (q * LPC);
         }
     }
@@ -421,8 +445,11 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
     if constexpr (SC) {
         g1_old = *J.g1;
         if (ctl.k > 0) {
+            const __amdgpu_buffer_rsrc_t fpr = buffer_over(J.f1_prev, (unsigned)N * 4u);
 #pragma unroll
-            for (int e = 0; e < VEC; e++) f1p[e] = rows[e] < N ? J.f1_prev[rows[e]] : 0.0f;
+            for (int e = 0; e < VEC; e++)  // rows past n_rows read 0
+                f1p[e] = __uint_as_float(
+                    __builtin_amdgcn_raw_buffer_load_b32(fpr, (int)((unsigned)rows[e] * 4u), 0, 0));
         }
     }
     ZMask smask = 0;  // screened steps of this column group (after the prologue)
@@ -464,11 +491,14 @@ __device__ __forceinline__ void qr_tile_lean(const QrTileJob &J, const IterCtl &
     };
     load_batch(0, xa, ca, ha);
     float h1[VEC];
+    const __amdgpu_buffer_rsrc_t h1r =
+        buffer_over(explicit_phi ? J.phi1 : J.ahat, (unsigned)N * 4u);
 #pragma unroll
-    for (int e = 0; e < VEC; e++)
-        h1[e] = rows[e] < N ? (explicit_phi ? J.phi1[rows[e]] : J.ahat[rows[e]]) : 0.0f;
+    for (int e = 0; e < VEC; e++)  // rows past n_rows read 0
+        h1[e] = __uint_as_float(
+            __builtin_amdgcn_raw_buffer_load_b32(h1r, (int)((unsigned)rows[e] * 4u), 0, 0));
 
-    const IterStart is = iteration_prologue(ctl, s_red, leader_block);
+    const IterStart is = iteration_prologue(ctl, s_red, leader_block, &pre);
     if (is.finish) return;
     const float gamma = is.gamma;
     const float s = J.step, kappa = J.kappa;
