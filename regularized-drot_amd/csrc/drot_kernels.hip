@@ -158,11 +158,6 @@ constexpr int kNormBatch = 8;  // LDS reads in flight per lane in the group norm
 // groups (copied at kernel start).
 constexpr int kGlBoundsLds = 1024;
 
-__device__ __forceinline__ void stage_bounds(const GlPassArgs &A, int *s_bnd) {
-    if (A.n_groups <= kGlBoundsLds)
-        for (int g = threadIdx.x; g <= A.n_groups; g += kBlock) s_bnd[g] = A.bounds[g];
-    __syncthreads();
-}
 
 // Fused GL-DROT pass for one tile of `tc` whole columns.  Restates
 // group_lasso_regularizer_update_x_for_small (core/kernel_glr.hpp:59-161) and
@@ -179,26 +174,45 @@ __global__ __launch_bounds__(kBlock) void gl_pass_kernel(GlPassArgs A) {
     extern __shared__ double s_dyn[];
     __shared__ double s_red[3 * kWavesPerBlock];
     __shared__ int s_bnd[kGlBoundsLds + 1];
+    __shared__ int s_done;
     const unsigned long long t0 = A.ctl.tspan ? (unsigned long long)wall_clock64() : 0ull;
-    if (A.ctl.st->done) return;
-    stage_bounds(A, s_bnd);
-    // the rows' phi numerators and groups are loaded before the prologue's
-    // reductions, so their round trip overlaps it
+    // Loads before the prologue, in this order and as raw buffer loads with an
+    // out-of-range offset for a lane with nothing to read (see gl_rows_kernel):
+    // the prologue's finalize entries, the group bounds, the rows' potentials
+    // and groups.
+    PrologueLoads pre;
+    prologue_issue(A.ctl, pre);
+    const int done_v = A.ctl.st->done;  // uniform: a scalar load
+    const int G0 = A.n_groups;
+    const bool bnd_lds0 = G0 <= kGlBoundsLds;
+    const __amdgpu_buffer_rsrc_t br =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<int *>(A.bounds), 0, (G0 + 1) * 4, 0x00020000);
+    const int bnd_v = (int)__builtin_amdgcn_raw_buffer_load_b32(
+        br, (int)(bnd_lds0 && (int)threadIdx.x <= G0 ? threadIdx.x * 4u : 0x80000000u), 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
     const bool explicit_phi = A.ctl.k == 0;
     const float *hsrc = explicit_phi ? A.phi1 : A.ahat;
+    const __amdgpu_buffer_rsrc_t h1r =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(hsrc), 0, A.n_rows * 4, 0x00020000);
+    const __amdgpu_buffer_rsrc_t gr = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<int *>(A.row_group), 0, A.n_rows * 4, 0x00020000);
     float h1[RCH][VEC];
     int gid[RCH][VEC];
 #pragma unroll
     for (int r = 0; r < RCH; r++) {
         const int i0 = (r * kBlock + (int)threadIdx.x) * VEC;
-#pragma unroll
-        for (int e = 0; e < VEC; e++) { h1[r][e] = 0.0f; gid[r][e] = 0; }
-        if (i0 < A.n_rows) {
-            VecIO<VEC>::load(hsrc + i0, h1[r]);
-            load_ints<VEC>(A.row_group + i0, gid[r]);
-        }
+        const unsigned off = i0 < A.n_rows ? (unsigned)i0 * 4u : 0x80000000u;
+        VecIO<VEC>::load_buf(h1r, off, h1[r]);
+        load_ints_buf<VEC>(gr, off, gid[r]);
     }
-    const IterStart is = iteration_prologue(A.ctl, s_red, blockIdx.x == 0);
+    if (threadIdx.x == 0) s_done = done_v;
+    if (bnd_lds0 && (int)threadIdx.x <= G0) s_bnd[threadIdx.x] = bnd_v;
+    if (bnd_lds0 && G0 >= kBlock) {  // more bounds than threads (rare): the rest directly
+        for (int g = threadIdx.x + kBlock; g <= G0; g += kBlock) s_bnd[g] = A.bounds[g];
+    }
+    __syncthreads();
+    if (s_done) return;  // the solve ended before this launch
+    const IterStart is = iteration_prologue(A.ctl, s_red, blockIdx.x == 0, &pre);
     if (is.finish) return;
     const int phi_mode = is.explicit_phi ? 1 : 0;
     const float gamma = is.gamma;
@@ -391,25 +405,37 @@ __global__ __launch_bounds__(kBlock) void gl_wave_kernel(GlPassArgs A) {
     __shared__ double s_red[3 * kWavesPerBlock];
     __shared__ double s_comb[2][kWavesPerBlock][kChunk];
     __shared__ int s_bnd[kGlBoundsLds + 1];
+    __shared__ int s_done;
     const unsigned long long t0 = A.ctl.tspan ? (unsigned long long)wall_clock64() : 0ull;
-    if (A.ctl.st->done) return;
-    stage_bounds(A, s_bnd);
-    // the rows' phi numerators and groups are loaded before the prologue's
-    // reductions, so their round trip overlaps it
+    // Loads before the prologue in this order, all raw buffer loads with an
+    // out-of-range offset for a lane with nothing to read (see gl_rows_kernel):
+    // the prologue's finalize entries, the group bounds, the rows' potentials
+    // and groups, the first column's X and C.
+    PrologueLoads pre;
+    prologue_issue(A.ctl, pre);
+    const int done_v = A.ctl.st->done;  // uniform: a scalar load
+    const int G0 = A.n_groups;
+    const bool bnd_lds0 = G0 <= kGlBoundsLds;
+    const __amdgpu_buffer_rsrc_t br =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<int *>(A.bounds), 0, (G0 + 1) * 4, 0x00020000);
+    const int bnd_v = (int)__builtin_amdgcn_raw_buffer_load_b32(
+        br, (int)(bnd_lds0 && (int)threadIdx.x <= G0 ? threadIdx.x * 4u : 0x80000000u), 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
     const int lane0 = threadIdx.x & (kWave - 1);
     const bool explicit_phi = A.ctl.k == 0;
     const float *hsrc = explicit_phi ? A.phi1 : A.ahat;
+    const __amdgpu_buffer_rsrc_t h1r =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(hsrc), 0, A.n_rows * 4, 0x00020000);
+    const __amdgpu_buffer_rsrc_t gr = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<int *>(A.row_group), 0, A.n_rows * 4, 0x00020000);
     float h1[RCH][VEC];
     int gid[RCH][VEC];
 #pragma unroll
     for (int r = 0; r < RCH; r++) {
         const int i0 = (r * kWave + lane0) * VEC;
-#pragma unroll
-        for (int e = 0; e < VEC; e++) { h1[r][e] = 0.0f; gid[r][e] = 0; }
-        if (i0 < A.n_rows) {
-            VecIO<VEC>::load(hsrc + i0, h1[r]);
-            load_ints<VEC>(A.row_group + i0, gid[r]);
-        }
+        const unsigned off = i0 < A.n_rows ? (unsigned)i0 * 4u : 0x80000000u;
+        VecIO<VEC>::load_buf(h1r, off, h1[r]);
+        load_ints_buf<VEC>(gr, off, gid[r]);
     }
     // this block's columns: [j0, j1); wave w takes j0 + w, j0 + w + 4, ...
     // The first column's X and C are loaded before the prologue as well.
@@ -418,24 +444,34 @@ __global__ __launch_bounds__(kBlock) void gl_wave_kernel(GlPassArgs A) {
     const int j1 = min(j0 + A.tc, A.n_cols);
     float xv[RCH][VEC], cv[RCH][VEC];
     float h2 = 0.0f;
-    auto load_col = [&](int j) {
-        const float *xc = A.x + (size_t)j * A.n_rows;
-        const float *cc = A.c + (size_t)j * A.n_rows;
-        h2 = explicit_phi ? A.phi2[j] : A.bhat[j];
+    const unsigned blk_bytes = (unsigned)(j1 - j0) * (unsigned)A.n_rows * 4u;
+    const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(
+        A.x + (size_t)j0 * A.n_rows, 0, (int)blk_bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t cr = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float *>(A.c) + (size_t)j0 * A.n_rows, 0, (int)blk_bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t h2r = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float *>(explicit_phi ? A.phi2 : A.bhat) + j0, 0, (j1 - j0) * 4, 0x00020000);
+    auto load_col = [&](int j) {  // j < j1
+        const unsigned jc = (unsigned)(j - j0);
+        h2 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(h2r, (int)(jc * 4u), 0, 0));
 #pragma unroll
         for (int r = 0; r < RCH; r++) {
             const int i0 = (r * kWave + lane0) * VEC;
-            if (i0 < A.n_rows) {
-                VecIO<VEC>::load(xc + i0, xv[r]);
-                VecIO<VEC>::load(cc + i0, cv[r]);
-            } else {
-#pragma unroll
-                for (int e = 0; e < VEC; e++) { xv[r][e] = 0.0f; cv[r][e] = 0.0f; }
-            }
+            const unsigned off =
+                i0 < A.n_rows ? (jc * (unsigned)A.n_rows + (unsigned)i0) * 4u : 0x80000000u;
+            VecIO<VEC>::load_buf(xr, off, xv[r]);
+            VecIO<VEC>::load_buf(cr, off, cv[r]);
         }
     };
     if (j0 + w < j1) load_col(j0 + w);
-    const IterStart is = iteration_prologue(A.ctl, s_red, blockIdx.x == 0);
+    if (threadIdx.x == 0) s_done = done_v;
+    if (bnd_lds0 && (int)threadIdx.x <= G0) s_bnd[threadIdx.x] = bnd_v;
+    if (bnd_lds0 && G0 >= kBlock) {  // more bounds than threads (rare): the rest directly
+        for (int g = threadIdx.x + kBlock; g <= G0; g += kBlock) s_bnd[g] = A.bounds[g];
+    }
+    __syncthreads();
+    if (s_done) return;  // the solve ended before this launch
+    const IterStart is = iteration_prologue(A.ctl, s_red, blockIdx.x == 0, &pre);
     if (is.finish) return;
     const int phi_mode = is.explicit_phi ? 1 : 0;
     const float gamma = is.gamma;
@@ -809,6 +845,9 @@ __global__ __launch_bounds__(kBlock) void gl_split_norm_kernel(GlPassArgs A) {
     __shared__ float s_y[R];
     __shared__ double s_red[3 * kWavesPerBlock];
     if (A.ctl.st->done) return;
+    PrologueLoads pre;  // the prologue's first entries go out first (see gl_rows_kernel)
+    prologue_issue(A.ctl, pre);
+    __builtin_amdgcn_sched_barrier(0);
     const int rc = blockIdx.x % A.n_rt;
     const int ct = blockIdx.x / A.n_rt;
     const int N = A.n_rows, M = A.n_cols;
@@ -820,7 +859,7 @@ __global__ __launch_bounds__(kBlock) void gl_split_norm_kernel(GlPassArgs A) {
     split_rows<VEC, RCH>(A, explicit_phi, r0, h1, gid);
     const int g_first = A.row_group[r0];
     const int g_last = A.row_group[r1 - 1];
-    const IterStart is = iteration_prologue(A.ctl, s_red, blockIdx.x == 0);
+    const IterStart is = iteration_prologue(A.ctl, s_red, blockIdx.x == 0, &pre);
     if (is.finish) return;
     const float gamma = is.gamma;
     const float fM = (float)M, fN = (float)N;

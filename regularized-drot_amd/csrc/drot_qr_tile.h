@@ -780,6 +780,11 @@ __device__ __forceinline__ void qr_tile_generic(const QrTileJob &J, const IterCt
     const bool explicit_phi = (ctl.k == 0);
     const float *hat2 = explicit_phi ? J.phi2 : J.bhat;
 
+    // the prologue's first entries go out before everything else (see qr_tile_lean)
+    PrologueLoads pre;
+    prologue_issue(ctl, pre);
+    __builtin_amdgcn_sched_barrier(0);
+
     // flags of this column group's segments from the previous pass: bit t is
     // column step t (loaded one per lane, gathered by a ballot).  A 64-lane
     // column group can hold up to 64 steps, so its mask is 64 bits wide.
@@ -863,7 +868,7 @@ __device__ __forceinline__ void qr_tile_generic(const QrTileJob &J, const IterCt
     __amdgpu_buffer_rsrc_t xrsrc = __builtin_amdgcn_make_buffer_rsrc(
         J.x + (size_t)tile_col0 * (size_t)ld, 0, tile_cols * ld * 4, kBufDword3);
 
-    const IterStart is = iteration_prologue(ctl, s_red, leader_block);
+    const IterStart is = iteration_prologue(ctl, s_red, leader_block, &pre);
     if (is.finish) return;
     const float gamma = is.gamma;
     const float s = J.step, kappa = J.kappa;
