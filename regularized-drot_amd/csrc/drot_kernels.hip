@@ -351,7 +351,8 @@ __device__ __forceinline__ void column_group_scales(const float *y, float *sc, i
     if (lane_per_group) {
         for (int g = lane; g < G; g += kWave) {
             const int hi = bnd(g + 1);
-            int i = bnd(g);
+            const int lo = bnd(g);
+            int i = lo;
             double n2 = 0.0;
             for (; i + kNormBatch <= hi; i += kNormBatch) {
                 float v[kNormBatch];
@@ -360,7 +361,19 @@ __device__ __forceinline__ void column_group_scales(const float *y, float *sc, i
 #pragma unroll
                 for (int u = 0; u < kNormBatch; u++) n2 += (double)v[u] * (double)v[u];
             }
-            for (; i < hi; i++) n2 += (double)y[i] * (double)y[i];
+            // the last rows (fewer than kNormBatch) read at once, clamped to
+            // the group's first row; a row past the group adds +0.0, which
+            // leaves the sum's bits unchanged
+            if (i < hi) {
+                float v[kNormBatch - 1];
+#pragma unroll
+                for (int u = 0; u < kNormBatch - 1; u++) v[u] = y[i + u < hi ? i + u : lo];
+#pragma unroll
+                for (int u = 0; u < kNormBatch - 1; u++) {
+                    const double d = (double)v[u];
+                    n2 += i + u < hi ? d * d : 0.0;
+                }
+            }
             sc[g] = group_scale(n2, lam);
         }
     } else {
@@ -369,7 +382,8 @@ __device__ __forceinline__ void column_group_scales(const float *y, float *sc, i
         double mine = 0.0;
         for (int g = 0; g < G; g++) {
             const int hi = bnd(g + 1);
-            int i = bnd(g) + lane;
+            const int lo = bnd(g);
+            int i = lo + lane;
             double n2 = 0.0;
             for (; i + (kNormBatch - 1) * kWave < hi; i += kNormBatch * kWave) {
                 float v[kNormBatch];
@@ -378,7 +392,19 @@ __device__ __forceinline__ void column_group_scales(const float *y, float *sc, i
 #pragma unroll
                 for (int u = 0; u < kNormBatch; u++) n2 += (double)v[u] * (double)v[u];
             }
-            for (; i < hi; i += kWave) n2 += (double)y[i] * (double)y[i];
+            if (i < hi) {  // the lane's last rows (fewer than kNormBatch) read at once, as above
+                float v[kNormBatch - 1];
+#pragma unroll
+                for (int u = 0; u < kNormBatch - 1; u++) {
+                    const int ii = i + u * kWave;
+                    v[u] = y[ii < hi ? ii : lo];
+                }
+#pragma unroll
+                for (int u = 0; u < kNormBatch - 1; u++) {
+                    const double d = (double)v[u];
+                    n2 += i + u * kWave < hi ? d * d : 0.0;
+                }
+            }
             n2 = wave_sum(n2);
             if ((g & (kWave - 1)) == lane) mine = n2;
             if ((g & (kWave - 1)) == kWave - 1 || g == G - 1) {
@@ -517,6 +543,13 @@ __global__ __launch_bounds__(kBlock) void gl_wave_kernel(GlPassArgs A) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // every scale read goes out at once (a row past n_rows has group 0)
+        float scv[RCH][VEC];
+#pragma unroll
+        for (int r = 0; r < RCH; r++) {
+#pragma unroll
+            for (int e = 0; e < VEC; e++) scv[r][e] = s_sc[gid[r][e]];
+        }
         double csum = 0.0;
 #pragma unroll
         for (int r = 0; r < RCH; r++) {
@@ -525,7 +558,7 @@ __global__ __launch_bounds__(kBlock) void gl_wave_kernel(GlPassArgs A) {
                 float xo[VEC];
 #pragma unroll
                 for (int e = 0; e < VEC; e++) {
-                    const float v = xv[r][e] * s_sc[gid[r][e]];
+                    const float v = xv[r][e] * scv[r][e];
                     xo[e] = v;
                     racc[r][e] += (double)v;
                     csum += (double)v;
@@ -711,12 +744,23 @@ __global__ __launch_bounds__(kBlock) void gl_rows_kernel(GlPassArgs A) {
             column_group_scales(ys, ys + N, G, A.lane_per_group != 0, bnd, lam, lane);
         }
         __syncthreads();
+        // every scale read of the round goes out at once (a row past n_rows
+        // has group 0; the slices of columns past j1 exist and are not used)
+        float scv[kCols][RCH][VEC];
+#pragma unroll
+        for (int jj = 0; jj < kCols; jj++) {
+            const float *sc = s_cols + jj * slice + N;
+#pragma unroll
+            for (int r = 0; r < RCH; r++) {
+#pragma unroll
+                for (int e = 0; e < VEC; e++) scv[jj][r][e] = sc[gid[r][e]];
+            }
+        }
         double cs[kCols];
 #pragma unroll
         for (int jj = 0; jj < kCols; jj++) {
             cs[jj] = 0.0;
             if (jr + jj >= j1) continue;  // block-uniform
-            const float *sc = s_cols + jj * slice + N;
             float *xo_col = A.x + (size_t)(jr + jj) * N;
 #pragma unroll
             for (int r = 0; r < RCH; r++) {
@@ -725,7 +769,7 @@ __global__ __launch_bounds__(kBlock) void gl_rows_kernel(GlPassArgs A) {
                     float xo[VEC];
 #pragma unroll
                     for (int e = 0; e < VEC; e++) {
-                        const float v = xv[jj][r][e] * sc[gid[r][e]];
+                        const float v = xv[jj][r][e] * scv[jj][r][e];
                         xo[e] = v;
                         racc[r][e] += (double)v;
                         cs[jj] += (double)v;
