@@ -19,6 +19,8 @@ namespace drot {
 
 // ----------------------------------------------------------------- helpers
 
+constexpr int kGlPolicySc1 = 16;  // cache-policy operand of a buffer store: sc1
+
 template <int VEC>
 struct VecIO;
 
@@ -39,6 +41,15 @@ struct VecIO<4> {
     static __device__ __forceinline__ void store(float *p, const float (&v)[4]) {
         *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
     }
+    // raw buffer store, device-scope write-through (sc1): the stored columns
+    // stay out of the L2 dirty set the kernel's end writes back
+    static __device__ __forceinline__ void store_wt(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                                    const float (&v)[4]) {
+        typedef unsigned int u4_t __attribute__((ext_vector_type(4)));
+        const u4_t d = {__float_as_uint(v[0]), __float_as_uint(v[1]), __float_as_uint(v[2]),
+                        __float_as_uint(v[3])};
+        __builtin_amdgcn_raw_buffer_store_b128(d, r, (int)off, 0, kGlPolicySc1);
+    }
 };
 
 template <>
@@ -49,6 +60,10 @@ struct VecIO<1> {
     }
     static __device__ __forceinline__ void load(const float *p, float (&v)[1]) { v[0] = *p; }
     static __device__ __forceinline__ void store(float *p, const float (&v)[1]) { *p = v[0]; }
+    static __device__ __forceinline__ void store_wt(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                                    const float (&v)[1]) {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v[0]), r, (int)off, 0, kGlPolicySc1);
+    }
 };
 
 // raw buffer load of VEC ints: an offset past the resource's range reads 0
@@ -564,7 +579,7 @@ __global__ __launch_bounds__(kBlock) void gl_wave_kernel(GlPassArgs A) {
                     csum += (double)v;
                     oacc += (double)v * (double)cv[r][e];
                 }
-                VecIO<VEC>::store(A.x + (size_t)j * N + i0, xo);
+                VecIO<VEC>::store_wt(xr, ((unsigned)(j - j0) * (unsigned)N + (unsigned)i0) * 4u, xo);
             }
         }
         csum = wave_sum(csum);
@@ -761,7 +776,7 @@ __global__ __launch_bounds__(kBlock) void gl_rows_kernel(GlPassArgs A) {
         for (int jj = 0; jj < kCols; jj++) {
             cs[jj] = 0.0;
             if (jr + jj >= j1) continue;  // block-uniform
-            float *xo_col = A.x + (size_t)(jr + jj) * N;
+            const unsigned col_off = (unsigned)(jr + jj - j0) * (unsigned)N;
 #pragma unroll
             for (int r = 0; r < RCH; r++) {
                 const int i0 = (r * kBlock + (int)threadIdx.x) * VEC;
@@ -775,7 +790,7 @@ __global__ __launch_bounds__(kBlock) void gl_rows_kernel(GlPassArgs A) {
                         cs[jj] += (double)v;
                         oacc += (double)v * (double)cv[jj][r][e];
                     }
-                    VecIO<VEC>::store(xo_col + i0, xo);
+                    VecIO<VEC>::store_wt(xr, (col_off + (unsigned)i0) * 4u, xo);
                 }
             }
         }
