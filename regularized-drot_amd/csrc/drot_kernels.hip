@@ -811,12 +811,28 @@ __global__ __launch_bounds__(kBlock) void gl_rows_kernel(GlPassArgs A) {
         // the next round writes s_cs only after two more barriers, and its
         // LDS slices only after every thread has passed this one
     }
+    // the block's row partials, written through like X (rows past n_rows
+    // take an out-of-range offset and are dropped)
+    const __amdgpu_buffer_rsrc_t pr = __builtin_amdgcn_make_buffer_rsrc(
+        A.prow + (size_t)blockIdx.x * N, 0, N * 8, 0x00020000);
 #pragma unroll
     for (int r = 0; r < RCH; r++) {
+        const int i0 = (r * kBlock + (int)threadIdx.x) * VEC;
+        const unsigned off = i0 < N ? (unsigned)i0 * 8u : 0x80000000u;
+        if constexpr (VEC == 4) {
+            typedef unsigned int u4_t __attribute__((ext_vector_type(4)));
 #pragma unroll
-        for (int e = 0; e < VEC; e++) {
-            const int i = (r * kBlock + (int)threadIdx.x) * VEC + e;
-            if (i < N) A.prow[(size_t)blockIdx.x * N + i] = racc[r][e];
+            for (int h = 0; h < 2; h++) {
+                const unsigned long long a = (unsigned long long)__double_as_longlong(racc[r][2 * h]);
+                const unsigned long long b = (unsigned long long)__double_as_longlong(racc[r][2 * h + 1]);
+                const u4_t d = {(unsigned)a, (unsigned)(a >> 32), (unsigned)b, (unsigned)(b >> 32)};
+                __builtin_amdgcn_raw_buffer_store_b128(d, pr, (int)(off + 16u * h), 0, kGlPolicySc1);
+            }
+        } else {
+            typedef unsigned int u2_t __attribute__((ext_vector_type(2)));
+            const unsigned long long a = (unsigned long long)__double_as_longlong(racc[r][0]);
+            const u2_t d = {(unsigned)a, (unsigned)(a >> 32)};
+            __builtin_amdgcn_raw_buffer_store_b64(d, pr, (int)off, 0, kGlPolicySc1);
         }
     }
     const double o = block_sum(oacc, s_red);
