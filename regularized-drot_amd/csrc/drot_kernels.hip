@@ -21,6 +21,14 @@ namespace drot {
 
 constexpr int kGlPolicySc1 = 16;  // cache-policy operand of a buffer store: sc1
 
+// one fp64 value through a buffer resource, device-scope write-through (sc1)
+__device__ __forceinline__ void store_f64_wt(__amdgpu_buffer_rsrc_t r, unsigned off, double v) {
+    typedef unsigned int u2_t __attribute__((ext_vector_type(2)));
+    const unsigned long long a = (unsigned long long)__double_as_longlong(v);
+    const u2_t d = {(unsigned)a, (unsigned)(a >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b64(d, r, (int)off, 0, kGlPolicySc1);
+}
+
 template <int VEC>
 struct VecIO;
 
@@ -593,6 +601,8 @@ __global__ __launch_bounds__(kBlock) void gl_wave_kernel(GlPassArgs A) {
     // chunk through two alternating LDS buffers (one barrier per chunk; the
     // barrier of chunk r + 1 orders the reads of chunk r before the writes of
     // chunk r + 2 into the same buffer)
+    const __amdgpu_buffer_rsrc_t pr = __builtin_amdgcn_make_buffer_rsrc(
+        A.prow + (size_t)blockIdx.x * N, 0, N * 8, 0x00020000);
 #pragma unroll
     for (int r = 0; r < RCH; r++) {
         if (r * kChunk >= N) break;
@@ -606,7 +616,7 @@ __global__ __launch_bounds__(kBlock) void gl_wave_kernel(GlPassArgs A) {
                 double sum = 0.0;
 #pragma unroll
                 for (int q = 0; q < kWavesPerBlock; q++) sum += cb[q][t];
-                A.prow[(size_t)blockIdx.x * N + i] = sum;
+                store_f64_wt(pr, (unsigned)i * 8u, sum);  // written through, as X
             }
         }
     }
